@@ -1,0 +1,172 @@
+#!/usr/bin/env python3
+"""Benchmark of the Cas-OFFinder search path on one MI355X.
+
+A step is one full search of the workload over the genome resident in HBM:
+key tables submitted -> fused finder+comparer kernels -> compact records in
+host memory (sorted).  Genome generation/upload and text formatting are not
+in the step and are reported separately.
+
+Default workload (genome_3g_ngg): synthetic 3.1 Gbp genome (seed 1, 24
+contigs, GC 41%, ~5% N), pattern 20N+NGG, 100 random 20-nt guides, mm <= 4.
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import sys
+import tempfile
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "cas-offinder_amd"))
+
+import casoff  # noqa: E402
+import workloads  # noqa: E402
+
+METRIC = "guide×Gbp/s bit-exact, batched vs plain on 1 GPU; serving p50/p99 latency"
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
+HBM_PEAK_GBS = 8000.0
+OPS_PER_BASE_CMP = 0.75  # cost model of a 4-bit one-hot compare: 6 VALU ops per 8 bases
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def build_workload(name, ctx, n_guides=None):
+    if name == "genome_3g_ngg":
+        texts = [workloads.genome_3g_ngg("synthetic", n_guides or 100)]
+        desc = "3.1 Gbp synthetic genome (seed 1, 24 contigs), pattern 20N+NGG, %d guides, mm<=4" % (n_guides or 100)
+    elif name == "bulge_3g_nrg":
+        texts = [workloads.bulge_3g_nrg("synthetic", n_guides or 20)]
+        desc = "3.1 Gbp synthetic genome (seed 1), 20N+NRG, DNA bulge 2 / RNA bulge 1, %d guides, mm<=4" % (n_guides or 20)
+    elif name == "mixed_pam_2000":
+        texts = [t for _, t in workloads.mixed_pam_2000("synthetic", n_guides or 2000)]
+        desc = "3.1 Gbp synthetic genome (seed 1), %d mixed-PAM guides (seed 2), one query per (PAM, D, R) group" % (n_guides or 2000)
+    else:
+        raise SystemExit("unknown workload " + name)
+    qs = [ctx.query(t) for t in texts]
+    guides = sum(len([l for l in t.split("\n")[2:] if l.strip()]) for t in texts)
+    return texts, qs, guides, desc
+
+
+def time_steps(ctx, qs, mode, steps, warmup):
+    for _ in range(warmup):
+        ctx.search(qs, mode)
+    kernel_ms, launches = 0.0, 0
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        r = ctx.search(qs, mode)
+        st = ctx.stats()
+        kernel_ms += st["t_fused_ms"] if mode == casoff.MODE_BATCHED else st["t_finder_ms"] + st["t_comparer_ms"]
+        launches += st["n_launches"]
+    t1 = time.perf_counter()
+    return (t1 - t0) / steps, kernel_ms / steps, launches / steps, st, r
+
+
+def cpu_baseline(ctx, texts, target_s=12.0):
+    """The oracle (CPU restatement, OpenMP) on a prefix of the first contig."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle
+    threads = min(os.cpu_count() or 1, 16)
+    name, pos, clen, _ = ctx.contigs()[0]
+    sample = min(clen, 4_000_000)
+    with tempfile.TemporaryDirectory() as td:
+        res = None
+        for _ in range(3):
+            gp = os.path.join(td, "sample.fa")
+            with open(gp, "w") as f:
+                f.write(">" + name + "\n")
+                seq = ctx.text(pos, sample)
+                for i in range(0, len(seq), 60):
+                    f.write(seq[i:i + 60] + "\n")
+            ip = os.path.join(td, "in.txt")
+            with open(ip, "w") as f:
+                f.write(gp + "\n" + texts[0].split("\n", 1)[1])
+            res = oracle.time_search(ip, threads)
+            if res["seconds"] >= target_s / 4 or sample >= clen:
+                break
+            sample = min(clen, int(sample * max(2.0, target_s / max(res["seconds"], 1e-3))))
+    n_guides = len([l for l in texts[0].split("\n")[2:] if l.strip()])
+    value = n_guides * (res["genome_len"] / 1e9) / res["seconds"]
+    return {"value": value, "unit": "guide×Gbp/s", "cores": threads, "kind": "port",
+            "sample": "oracle (reference-algorithm CPU restatement, OpenMP) on the first %d bp of %s with the "
+                      "first query's %d guides: %.2f s, %d candidates" % (sample, name, n_guides, res["seconds"],
+                                                                          res["candidates"])}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="genome_3g_ngg")
+    ap.add_argument("--guides", type=int, default=None)
+    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--genome-len", type=int, default=3_100_000_000)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding).",
+              file=sys.stderr)
+        sys.exit(2)
+
+    ctx = casoff.Context(0)
+    t0 = time.perf_counter()
+    ctx.synth(workloads.genome_3g(total_len=args.genome_len))
+    t_load = time.perf_counter() - t0
+    total, nfiles, ncontigs = ctx.info()
+    log("genome: %d positions, %d contigs, generated+uploaded in %.1f s" % (total, ncontigs, t_load))
+    texts, qs, n_guides, desc = build_workload(args.workload, ctx, args.guides)
+
+    s_b, k_b, l_b, st_b, r_b = time_steps(ctx, qs, casoff.MODE_BATCHED, args.steps, args.warmup)
+    hits_b = r_b.records()
+    log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
+    s_p, k_p, l_p, st_p, r_p = time_steps(ctx, qs, casoff.MODE_PLAIN, args.plain_steps, 0)
+    hits_p = r_p.records()
+    log("plain: %.2f ms/step (kernels %.2f ms), %d hits" % (s_p * 1e3, k_p, len(hits_p)))
+    same = len(hits_b) == len(hits_p) and (hits_b == hits_p).all()
+
+    gbp = total / 1e9
+    value = n_guides * gbp / s_b
+    plain_value = n_guides * gbp / s_p
+    avg_launch_ms = k_b / max(l_b, 1)
+    base_cmp = st_b["n_base_cmp"] / max(l_b, 1)
+    achieved = OPS_PER_BASE_CMP * base_cmp / (avg_launch_ms * 1e-3) / 1e12
+    hbm_achieved = 0.375 * total / (avg_launch_ms * 1e-3) / 1e9
+    out = {
+        "metric": METRIC,
+        "value": round(value, 2),
+        "unit": "guide×Gbp/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(s_b * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "u32",
+        "data": "synthetic",
+        "config": {"workload": args.workload + ": " + desc, "guides": n_guides, "genome_bp": total,
+                   "mode": "batched (fused finder+comparer, one launch per query)"},
+        "plain": {"value": round(plain_value, 2), "ms_per_step": round(s_p * 1e3, 3), "steps": args.plain_steps,
+                  "launches_per_step": l_p, "bit_exact_vs_batched": bool(same)},
+        "batched_over_plain": round(value / plain_value, 2),
+        "candidate_sites_per_s": st_b["n_pairs"] / s_b,
+        "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),
+                     "base_comparisons": st_b["n_base_cmp"]},
+        "genome_generate_upload_s": round(t_load, 2),
+        "roofline": {"bound": "valu", "kernel": "fused_kernel", "achieved": round(achieved, 3),
+                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
+                     "avg_launch_ms": round(avg_launch_ms, 4), "traffic": None,
+                     "hbm_equiv": {"achieved": round(hbm_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                   "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(ctx, texts)
+    print(json.dumps(out), flush=True)
+    ctx.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,273 @@
+// extern "C" entry points of libcasoff.so (declared in include/casoff.h).
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "engine.hpp"
+#include "format.hpp"
+
+using namespace cof;
+
+static thread_local std::string g_noctx_err;
+
+extern "C" {
+
+const char *cof_version(void) { return "3.0.0-mi355x"; }
+
+int cof_open(int device, cof_ctx **out) {
+    if (!out) return COF_E_INVALID;
+    *out = nullptr;
+    if (device == -1) {  // host-only context
+        auto *c = new cof_ctx();
+        c->device = -1;
+        *out = c;
+        return COF_OK;
+    }
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) {
+        g_noctx_err = "No HIP devices found.";
+        return COF_E_DEVICE;
+    }
+    if (device < 0 || device >= n) {
+        g_noctx_err = "Device index out of range.";
+        return COF_E_INVALID;
+    }
+    if (hipSetDevice(device) != hipSuccess) return COF_E_DEVICE;
+    auto *c = new cof_ctx();
+    c->device = device;
+    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return COF_E_DEVICE; }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) { delete c; return COF_E_DEVICE; }
+    *out = c;
+    return COF_OK;
+}
+
+int cof_close(cof_ctx *c) {
+    if (!c) return COF_E_INVALID;
+    if (c->device < 0) {
+        delete c;
+        return COF_OK;
+    }
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    c->d_lo.release(); c->d_hi.release(); c->d_ot.release(); c->d_poison.release();
+    c->d_hits.release(); c->d_count.release(); c->d_loci.release(); c->d_flags.release();
+    c->d_keys.release(); c->d_scratch.release();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(c->stream);
+    delete c;
+    return COF_OK;
+}
+
+const char *cof_last_error(const cof_ctx *c) { return c ? c->err.c_str() : g_noctx_err.c_str(); }
+
+int cof_genome_load(cof_ctx *c, const char *path, int *nfiles) {
+    if (!c || !path) return COF_E_INVALID;
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    int nf = 0;
+    std::string err;
+    const char *v = getenv("COF_VERBOSE");
+    int rc = load_path(path, c->genome, nf, err, v && atoi(v) > 0);
+    if (rc) { c->genome_ready = false; return c->fail(rc, err); }
+    if (nfiles) *nfiles = nf;
+    return c->device < 0 ? COF_OK : c->upload_genome();
+}
+
+int cof_genome_synth(cof_ctx *c, const cof_synth_spec *spec) {
+    if (!c || !spec) return COF_E_INVALID;
+    if (c->device >= 0) (void)hipSetDevice(c->device);
+    synth_genome(*spec, c->genome);
+    return c->device < 0 ? COF_OK : c->upload_genome();
+}
+
+int cof_genome_info(const cof_ctx *c, uint64_t *total_len, uint64_t *n_files, uint64_t *n_contigs) {
+    if (!c) return COF_E_INVALID;
+    if (total_len) *total_len = c->genome.total;
+    if (n_files) *n_files = c->genome.files.size();
+    if (n_contigs) *n_contigs = c->genome.n_contigs();
+    return COF_OK;
+}
+
+int cof_genome_contig(const cof_ctx *c, uint64_t i, const char **name, uint64_t *pos, uint64_t *len,
+                      uint32_t *file) {
+    if (!c) return COF_E_INVALID;
+    for (size_t f = 0; f < c->genome.files.size(); f++) {
+        const GenomeFile &gf = c->genome.files[f];
+        if (i < gf.names.size()) {
+            const uint64_t b = gf.chrpos[i];
+            const uint64_t e = i + 1 < gf.names.size() ? gf.chrpos[i + 1] - 1 : gf.len;
+            if (name) *name = gf.names[i].c_str();
+            if (pos) *pos = gf.base + b;
+            if (len) *len = e >= b ? e - b : 0;
+            if (file) *file = (uint32_t)f;
+            return COF_OK;
+        }
+        i -= gf.names.size();
+    }
+    return COF_E_INVALID;
+}
+
+int cof_genome_text(const cof_ctx *c, uint64_t pos, uint64_t n, char *out) {
+    if (!c || !out || pos + n > c->genome.total) return COF_E_INVALID;
+    c->genome.text(pos, n, out);
+    return COF_OK;
+}
+
+int cof_synth_write(const cof_synth_spec *spec, const char *path, int format) {
+    if (!spec || !path || (format != 0 && format != 1)) return COF_E_INVALID;
+    std::string err;
+    int rc = synth_write(*spec, path, format, err);
+    if (rc) g_noctx_err = err;
+    return rc;
+}
+
+int cof_query_parse(cof_ctx *c, const char *text, size_t len, cof_query **out) {
+    if (!out || (!text && len)) return COF_E_INVALID;
+    *out = nullptr;
+    auto q = std::make_shared<Query>();
+    std::string err;
+    if (parse_query(text ? text : "", len, *q, err) || encode_query(*q, err)) {
+        if (c) c->err = err;
+        else g_noctx_err = err;
+        return COF_E_INVALID;
+    }
+    *out = new cof_query{q};
+    return COF_OK;
+}
+
+int cof_query_free(cof_query *q) {
+    delete q;
+    return COF_OK;
+}
+
+int cof_query_dump(const cof_query *q, char **out, size_t *len) {
+    if (!q || !out || !len) return COF_E_INVALID;
+    const Query &qq = *q->q;
+    std::string t = std::to_string(qq.P) + " " + std::to_string(qq.D) + " " + std::to_string(qq.R) + " " +
+                    qq.pattern + "\n";
+    for (auto &k : qq.keys) {
+        t += k.seq + "\t" + k.id + "\t" + std::to_string(k.threshold) + "\t";
+        for (size_t j = 0; j < k.infos.size(); j++) {
+            if (j) t += ",";
+            t += k.infos[j].first + ":" + std::to_string(k.infos[j].second);
+        }
+        t += "\n";
+    }
+    char *p = (char *)malloc(t.size() + 1);
+    if (!p) return COF_E_NOMEM;
+    memcpy(p, t.data(), t.size() + 1);
+    *out = p;
+    *len = t.size();
+    return COF_OK;
+}
+
+int cof_query_info(const cof_query *q, const char **chrdir, uint32_t *pattern_len, uint32_t *n_keys) {
+    if (!q) return COF_E_INVALID;
+    if (chrdir) *chrdir = q->q->chrdir.c_str();
+    if (pattern_len) *pattern_len = q->q->P;
+    if (n_keys) *n_keys = (uint32_t)q->q->keys.size();
+    return COF_OK;
+}
+
+int cof_search(cof_ctx *c, cof_query *const *queries, int n_queries, int mode, cof_result **out) {
+    if (!c || !out || n_queries < 0 || (n_queries && !queries)) return COF_E_INVALID;
+    if (c->device < 0) return c->fail(COF_E_UNSUPPORTED, "Host-only context cannot search.");
+    (void)hipSetDevice(c->device);
+    std::vector<std::shared_ptr<Query>> qs;
+    for (int i = 0; i < n_queries; i++) {
+        if (!queries[i]) return c->fail(COF_E_INVALID, "null query");
+        qs.push_back(queries[i]->q);
+    }
+    return c->search(qs, mode, out);
+}
+
+int cof_stats_get(const cof_ctx *c, cof_stats *out) {
+    if (!c || !out) return COF_E_INVALID;
+    *out = c->stats;
+    return COF_OK;
+}
+
+int64_t cof_result_count(const cof_result *r) { return r ? (int64_t)r->hits.size() : COF_E_INVALID; }
+
+int cof_result_records(const cof_result *r, cof_hit *out, size_t cap) {
+    if (!r || (!out && cap)) return COF_E_INVALID;
+    const size_t n = std::min(cap, r->hits.size());
+    if (n) memcpy(out, r->hits.data(), n * sizeof(cof_hit));
+    return COF_OK;
+}
+
+static int format_query(cof_ctx *c, const cof_result *r, int qi, std::string &text, SummaryTable *sum) {
+    if (!c || !r || qi < 0 || (size_t)qi >= r->queries.size()) return COF_E_INVALID;
+    const size_t b = r->qbegin[qi], e = r->qbegin[qi + 1];
+    static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
+    format_hits(c->genome, *r->queries[qi], reinterpret_cast<const DevHit *>(r->hits.data()) + b, e - b, text, sum);
+    return COF_OK;
+}
+
+int cof_result_format(cof_ctx *c, const cof_result *r, int qi, const char *path, int append,
+                      const char *summary_path) {
+    if (!path) return COF_E_INVALID;
+    std::string text;
+    SummaryTable sum;
+    const bool want_sum = summary_path && summary_path[0];
+    if (!append) text = header_lines();
+    int rc = format_query(c, r, qi, text, want_sum ? &sum : nullptr);
+    if (rc) return rc;
+    const bool to_stdout = strcmp(path, "-") == 0;
+    FILE *f = to_stdout ? stdout : fopen(path, append ? "ab" : "wb");
+    if (!f) return c->fail(COF_E_IO, std::string("cannot open ") + path);
+    fwrite(text.data(), 1, text.size(), f);
+    if (to_stdout) fflush(f);
+    else if (fclose(f) != 0) return c->fail(COF_E_IO, std::string("write failed: ") + path);
+    if (want_sum) {
+        std::string s = summary_text(sum);
+        const bool so = strcmp(summary_path, "-") == 0;
+        FILE *g = so ? stdout : fopen(summary_path, "wb");
+        if (!g) return c->fail(COF_E_IO, std::string("cannot open ") + summary_path);
+        fwrite(s.data(), 1, s.size(), g);
+        if (so) fflush(g);
+        else fclose(g);
+    }
+    return COF_OK;
+}
+
+int cof_result_format_mem(cof_ctx *c, const cof_result *r, int qi, char **out, size_t *len) {
+    if (!out || !len) return COF_E_INVALID;
+    std::string text;
+    int rc = format_query(c, r, qi, text, nullptr);
+    if (rc) return rc;
+    char *p = (char *)malloc(text.size() + 1);
+    if (!p) return COF_E_NOMEM;
+    memcpy(p, text.data(), text.size());
+    p[text.size()] = 0;
+    *out = p;
+    *len = text.size();
+    return COF_OK;
+}
+
+int cof_format_hits(cof_ctx *c, const cof_query *q, const cof_hit *hits, size_t n, char **out, size_t *len) {
+    if (!c || !q || !out || !len || (n && !hits)) return COF_E_INVALID;
+    const Query &qq = *q->q;
+    for (size_t i = 0; i < n; i++)
+        if (hits[i].key >= qq.keys.size() || hits[i].pos + qq.P > c->genome.total)
+            return c->fail(COF_E_INVALID, "hit out of range");
+    std::string text;
+    format_hits(c->genome, qq, reinterpret_cast<const DevHit *>(hits), n, text, nullptr);
+    char *p = (char *)malloc(text.size() + 1);
+    if (!p) return COF_E_NOMEM;
+    memcpy(p, text.data(), text.size());
+    p[text.size()] = 0;
+    *out = p;
+    *len = text.size();
+    return COF_OK;
+}
+
+int cof_result_free(cof_result *r) {
+    delete r;
+    return COF_OK;
+}
+
+void cof_free(void *p) { free(p); }
+
+}  // extern "C"

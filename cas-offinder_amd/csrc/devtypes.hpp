@@ -1,0 +1,53 @@
+// Types shared by the host engine and the gfx950 kernels.
+#pragma once
+
+#include <cstdint>
+
+namespace cof {
+
+// Base classes as bit indices of a 5-bit class set.  Codes 0..3 are the UCSC
+// 2bit codes (T=0, C=1, A=2, G=3); class 4 is any non-ACGT byte (N, ';', ...).
+enum : uint32_t { kClsT = 0, kClsC = 1, kClsA = 2, kClsG = 3, kClsO = 4 };
+
+constexpr int kMaxPattern = 64;   // longest padded pattern the kernels take
+constexpr int kTileWords = 256;   // one 32-position word per thread of a 256-thread workgroup
+constexpr int kTilePos = kTileWords * 32;
+constexpr int kHaloL = 2;         // words staged before the tile
+constexpr int kHaloR = 6;         // words staged after the tile
+constexpr int kLdsWords = kTileWords + kHaloL + kHaloR;
+
+// Finder pattern: the positions of the padded pattern (forward, and its
+// reverse complement in forward text orientation) that can mismatch, each with
+// the 5-bit set of classes that MATCH there.
+struct PatternDesc {
+    uint32_t P;
+    uint32_t nf, nr;
+    uint32_t pad;
+    uint8_t fk[kMaxPattern], fm[kMaxPattern];
+    uint8_t rk[kMaxPattern], rm[kMaxPattern];
+};
+
+// Compare key for the kernels: for each class, the key positions where a base
+// of that class counts as a mismatch, as bit b <-> key position cbase + b,
+// `NW` 32-bit words per class.  Stored as 8 dwords (NW = 1) or 16 dwords
+// (NW = 2): m[class][word] at class * NW + word, threshold at 5 * NW.
+constexpr int key_stride(int nw) { return nw == 1 ? 8 : 16; }
+
+// Device hit record, identical to cof_hit of casoff.h.
+struct DevHit {
+    uint64_t pos;
+    uint32_t key;
+    uint16_t mm;
+    uint8_t strand;
+    uint8_t query;
+};
+
+struct DevGenome {
+    const uint32_t *lo, *hi, *ot;  // planes, nwords + padding words
+    const uint64_t *poison;        // sorted; poison[npoison] is a sentinel >= total + kMaxPattern
+    uint64_t npoison;
+    uint64_t total;
+    uint64_t nwords;
+};
+
+}  // namespace cof

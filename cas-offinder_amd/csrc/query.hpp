@@ -1,0 +1,53 @@
+// Query = one reference input file: genome path, padded pattern with DNA/RNA
+// bulge sizes and the compare keys expanded from the guides
+// (cas-offinder.cpp:590-694), plus the device encodings of pattern and keys.
+#pragma once
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "devtypes.hpp"
+
+namespace cof {
+
+struct BulgeInfo {
+    std::string first;  // "0", the DNA bulge size, or the RNA-bulge bases
+    int second;         // bulge index, negated for a reversed PAM
+};
+
+struct CompareKey {
+    std::string seq;
+    std::string id;
+    uint16_t threshold = 0;
+    std::vector<BulgeInfo> infos;
+};
+
+struct Query {
+    std::string chrdir;
+    std::string pattern;  // padded, uppercase
+    uint32_t P = 0, D = 0, R = 0;
+    std::vector<CompareKey> keys;  // bytewise key order (std::map order)
+
+    // device encodings (encode_query)
+    PatternDesc pat{};
+    int nw = 1;
+    int cbase = 0;
+    std::vector<uint32_t> key_words;  // keys.size() * key_stride(nw)
+    uint64_t guides = 0;              // guide lines read
+    uint64_t care_sum = 0;            // sum over keys of positions that can mismatch
+};
+
+// 5-bit set of base classes that count as a mismatch against pattern/key char
+// `p` (bit kClsT..kClsO).  IUPAC (README.md:139-151): a concrete base
+// mismatches every other class including non-ACGT bytes; a degenerate code
+// mismatches exactly the concrete bases outside its set; N and any other byte
+// never mismatch.
+uint32_t mismatch_classes(char p);
+uint32_t class_of(char genome_char);
+void revcomp(std::string &s);  // cas-offinder.cpp:35-58 complement table, then reverse
+
+int parse_query(const char *text, size_t len, Query &q, std::string &err);
+int encode_query(Query &q, std::string &err);
+
+}  // namespace cof

@@ -1,0 +1,171 @@
+/*
+ * casoff.h -- C ABI of libcasoff.so, the MI355X (gfx950) Cas-OFFinder search engine.
+ *
+ * The reference (Cas-OFFinder 3.0.0) has no library API: its search path sits
+ * behind the Cas_OFFinder class (cas-offinder.h:84-106) and the fail-fast OpenCL
+ * wrappers (oclfunctions.h:20-133).  Each entry point below names the reference
+ * member or call sequence it replaces.  Conventions:
+ *   - every function returns int: 0 ok, < 0 error (COF_E_*); nothing calls exit();
+ *   - the library owns device memory and results; callers release with *_free;
+ *   - a context is single-threaded (one HIP stream); cof_last_error() holds the
+ *     message of the last failing call on it;
+ *   - positions are 0-based; strands are '+' / '-'.
+ */
+#ifndef CASOFF_H
+#define CASOFF_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define COF_OK 0
+#define COF_E_INVALID (-1)   /* malformed input, bad argument */
+#define COF_E_NOMEM (-2)     /* host or device allocation failed, admission refused */
+#define COF_E_DEVICE (-3)    /* HIP runtime error, no device */
+#define COF_E_IO (-4)        /* genome path unreadable */
+#define COF_E_UNSUPPORTED (-5)
+
+#define COF_MODE_PLAIN 0     /* finder launch, then one comparer launch per compare key */
+#define COF_MODE_BATCHED 1   /* fused finder+comparer over all keys, one launch per query */
+
+typedef struct cof_ctx cof_ctx;
+typedef struct cof_query cof_query;
+typedef struct cof_result cof_result;
+typedef struct cof_server cof_server;
+
+/* One device-side hit: the window start in the context's global genome
+ * coordinate, the compare key index (std::map order of the query's keys) and
+ * the mismatch count.  cof_result_records() returns these sorted by
+ * (query, key, pos, strand). */
+typedef struct {
+    uint64_t pos;
+    uint32_t key;
+    uint16_t mm;
+    uint8_t strand;   /* 0 '+', 1 '-' */
+    uint8_t query;    /* index of the query in the cof_search batch */
+} cof_hit;
+
+/* Per-search timing and work counters of the last cof_search on a context. */
+typedef struct {
+    double t_total_ms;      /* submission to compact records in host memory */
+    double t_finder_ms;     /* plain mode: finder kernels (HIP events) */
+    double t_comparer_ms;   /* plain mode: all comparer launches; batched: fused kernels */
+    double t_fused_ms;      /* batched mode: fused kernels (HIP events) */
+    uint64_t n_candidates;  /* (locus, strand) candidates that passed the PAM test */
+    uint64_t n_pairs;       /* candidate x compare-key evaluations */
+    uint64_t n_hits;
+    uint64_t n_launches;
+    uint64_t genome_len;    /* positions scanned (global coordinate incl. separators) */
+    uint64_t n_base_cmp;    /* sum over pairs of the key's comparable (non-N) positions */
+} cof_stats;
+
+/* Synthetic genome specification (seeded, deterministic; no reference
+ * counterpart -- inputs for tests and benchmarks). */
+typedef struct {
+    uint64_t seed;
+    uint32_t n_contigs;
+    uint32_t length_mode;    /* 0: weights U(0.5,1.5) scaled to total_len; 1: log-uniform in [min_len,max_len] scaled to total_len */
+    uint64_t total_len;
+    uint64_t min_len, max_len;
+    double gc;               /* fraction of G+C */
+    double n_frac;           /* fraction of positions covered by N runs */
+    uint32_t n_run_min, n_run_max;
+    double lower_frac;       /* fraction in soft-masked (lowercase) runs, FASTA output only */
+    uint32_t line_width;     /* FASTA line width */
+} cof_synth_spec;
+
+/* ---- context: replaces Cas_OFFinder(devtype, devarg) + initOpenCLDevices
+ *      (cas-offinder.cpp:80-156) and ~Cas_OFFinder (:158-173).  device -1
+ *      opens a host-only context (genome loading, parsing, formatting; no
+ *      search). */
+int cof_open(int device, cof_ctx **out);
+int cof_close(cof_ctx *ctx);
+const char *cof_last_error(const cof_ctx *ctx);
+const char *cof_version(void);
+
+/* ---- genome: replaces read_fasta (read_fasta.cpp:11-57) / read_twobit
+ *      (read_twobit.cpp:44-131) as driven by main.cpp:43-58 and the per-chunk
+ *      upload setChrData/loadNextChunk (cas-offinder.cpp:175-252).  `path` is a
+ *      directory (every regular file or symlink in readdir order, non-FASTA /
+ *      non-2bit files skipped) or one file.  The whole genome is packed as
+ *      2-bit bases + a non-ACGT bit plane and stays resident on the device
+ *      until the next load or cof_close.  Returns the number of files loaded
+ *      in *nfiles (may be NULL). */
+int cof_genome_load(cof_ctx *ctx, const char *path, int *nfiles);
+/* Generates the genome directly into the packed resident form. */
+int cof_genome_synth(cof_ctx *ctx, const cof_synth_spec *spec);
+/* Total positions (global coordinate, separators included), file and contig counts. */
+int cof_genome_info(const cof_ctx *ctx, uint64_t *total_len, uint64_t *n_files, uint64_t *n_contigs);
+/* Contig i (0-based over all files): name, global position of its first
+ * base, length, and the file it came from. */
+int cof_genome_contig(const cof_ctx *ctx, uint64_t i, const char **name, uint64_t *pos, uint64_t *len,
+                      uint32_t *file);
+/* Copies the genome text of [pos, pos+n) as the reference holds it
+ * (uppercase, ';' between contigs). */
+int cof_genome_text(const cof_ctx *ctx, uint64_t pos, uint64_t n, char *out);
+/* Writes a synthetic genome as FASTA (format 0) or UCSC 2bit (format 1). */
+int cof_synth_write(const cof_synth_spec *spec, const char *path, int format);
+
+/* ---- query: replaces Cas_OFFinder::readInputFile / parseInput
+ *      (cas-offinder.cpp:590-751): pattern line with DNA/RNA bulge sizes,
+ *      guides with thresholds and ids, expanded into compare keys in std::map
+ *      order with first-(id,threshold)-wins.  On error returns COF_E_INVALID
+ *      and the reference's message ("Malformed input file." ...) in
+ *      cof_last_error. */
+int cof_query_parse(cof_ctx *ctx, const char *text, size_t len, cof_query **out);
+int cof_query_free(cof_query *q);
+/* Diagnostic text of the parsed query (no reference counterpart): line
+ * "P D R pattern", then "key<TAB>id<TAB>threshold<TAB>first:second,..." per
+ * compare key in map order.  Caller frees *out with cof_free. */
+int cof_query_dump(const cof_query *q, char **out, size_t *len);
+/* Genome path (first input line), padded pattern length, key count. */
+int cof_query_info(const cof_query *q, const char **chrdir, uint32_t *pattern_len, uint32_t *n_keys);
+
+/* ---- search: replaces findPattern (cas-offinder.cpp:254-297) and the launch
+ *      part of compareAll (:356-393) for every chunk (main.cpp:58-65).
+ *      `queries` run over the resident genome; mode COF_MODE_PLAIN keeps the
+ *      reference's launch model (finder, then one comparer launch per key),
+ *      COF_MODE_BATCHED runs one fused finder+comparer launch per query. */
+int cof_search(cof_ctx *ctx, cof_query *const *queries, int n_queries, int mode, cof_result **out);
+int cof_stats_get(const cof_ctx *ctx, cof_stats *out);
+
+/* ---- results */
+int64_t cof_result_count(const cof_result *r);
+int cof_result_records(const cof_result *r, cof_hit *out, size_t cap);
+/* Writes the records of query `qi` as the reference's compareAll does
+ * (cas-offinder.cpp:394-456): one TSV line per (hit, bulge info).  `path` "-"
+ * is stdout; `append` 0 truncates and writes the two header lines of
+ * writeHeaders (:319-328) first.  When `summary_path` is non-empty the
+ * --summary table (writeSummaryTable, :471-500) is written there. */
+int cof_result_format(cof_ctx *ctx, const cof_result *r, int qi, const char *path, int append,
+                      const char *summary_path);
+/* Same records into a malloc-ed buffer (caller frees with cof_free). */
+int cof_result_format_mem(cof_ctx *ctx, const cof_result *r, int qi, char **out, size_t *len);
+int cof_result_free(cof_result *r);
+/* Formats caller-supplied hits of one query (hit.query ignored) against the
+ * context's genome, exactly as cof_result_format_mem; no header lines. */
+int cof_format_hits(cof_ctx *ctx, const cof_query *q, const cof_hit *hits, size_t n, char **out, size_t *len);
+void cof_free(void *p);
+
+/* ---- serving (no reference counterpart: the reference runs one input per
+ *      process).  A server owns the context on its own thread, admits queued
+ *      queries while the device-memory budget for key tables and hit buffers
+ *      allows, and merges admitted queries into batched passes as they come. */
+typedef struct {
+    uint64_t mem_budget_bytes;   /* key tables + hit buffers of in-flight queries */
+    uint32_t max_batch_queries;  /* queries merged into one pass */
+} cof_server_cfg;
+int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out);
+/* Takes ownership of q.  *ticket identifies the request. */
+int cof_server_submit(cof_server *s, cof_query *q, uint64_t *ticket);
+/* 0: done (*out set, latency in *latency_ms), 1: pending, <0 error. */
+int cof_server_poll(cof_server *s, uint64_t ticket, int wait, cof_result **out, double *latency_ms);
+int cof_server_stop(cof_server *s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,158 @@
+"""GPU parity: the HIP path (plain and batched) against the CPU oracle, record
+multisets compared bytewise after a canonical sort."""
+import os
+import random
+import subprocess
+
+import pytest
+
+import casoff
+import helpers as H
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+MODES = (casoff.MODE_PLAIN, casoff.MODE_BATCHED)
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = casoff.Context(0)
+    yield c
+    c.close()
+
+
+def product_lines(ctx, genome_path, text, mode):
+    ctx.load(genome_path)
+    q = ctx.query(text)
+    r = ctx.search([q], mode)
+    return H.record_lines(r.format(0))
+
+
+def check(ctx, tmp_path, genome_path, text, label=""):
+    exp = H.record_lines(H.oracle_records(tmp_path, text))
+    for mode in MODES:
+        got = product_lines(ctx, genome_path, text, mode)
+        assert got == exp, "%s mode=%d: %d vs %d records" % (label, mode, len(got), len(exp))
+    return exp
+
+
+def test_plain_10mbp_config(ctx, tmp_path):
+    gp = str(tmp_path / "g10.fa")
+    casoff.synth_write(workloads.genome_10mbp(), gp, 0)
+    exp = check(ctx, tmp_path, gp, workloads.plain_10mbp(gp), "10mbp")
+    assert len(exp) > 0
+
+
+PATTERNS = [
+    ("N" * 21 + "RG", False),
+    ("N" * 20 + "NGG", False),
+    ("N" * 21 + "NNGRRT", False),
+    ("N" * 20 + "NNAGAAW", False),
+    ("N" * 24 + "NNNNGMTT", False),
+    ("TTTV" + "N" * 23, True),
+    ("N" * 12, False),
+    ("N" * 40 + "NGMTT", False),          # care span > 32 -> two-word kernels
+    ("TTN" + "N" * 48, True),
+]
+
+
+def random_case(seed, tmp_path):
+    r = random.Random(seed)
+    pattern, rev = PATTERNS[seed % len(PATTERNS)]
+    D = r.choice([0, 0, 1, 2])
+    R = r.choice([0, 0, 1, 2])
+    P = len(pattern)
+    recs = []
+    for ci in range(r.randint(1, 4)):
+        n = r.choice([0, 5, P - 1, P, r.randint(50, 3000)])
+        alpha = r.choice(["ACGT", "ACGT", "ACGTN", "ACGTRYX"])
+        recs.append(("chr%d %s" % (ci, r.choice(["", "desc"])), H.random_seq(r, n, alpha, n_runs=r.randint(0, 2))))
+    guides = []
+    allseq = "".join(s for _, s in recs)
+    for gi in range(r.randint(1, 6)):
+        if len(allseq) >= P and r.random() < 0.8:
+            x = r.randrange(0, len(allseq) - P + 1)
+            g = list(allseq[x:x + P])
+            if r.random() < 0.5:
+                g = list(H.revcomp("".join(g)))
+        else:
+            g = [r.choice("ACGT") for _ in range(P)]
+        for _ in range(r.randint(0, 3)):
+            g[r.randrange(P)] = r.choice("ACGTRN")
+        g = ["N" if pc != "N" else (gc if gc in "ACGTRYSWKMBDHV" else "A") for pc, gc in zip(pattern, g)]
+        if r.random() < 0.2:
+            g = [c.lower() for c in g]
+        guides.append("".join(g))
+    if r.random() < 0.3 and guides:
+        guides.append(guides[0])  # duplicate guide: first id / threshold wins
+    lines = [pattern if (D == 0 and R == 0 and r.random() < 0.5) else "%s %d %d" % (pattern, D, R)]
+    for i, g in enumerate(guides):
+        lines.append("%s %d s%d" % (g, r.randint(0, 5), i))
+    fmt = r.choice(["fa", "fa", "2bit", "dir"])
+    if fmt == "fa":
+        gp = str(tmp_path / ("c%d.fa" % seed))
+        H.write_fasta(gp, recs, width=r.choice([7, 60, 1000]), crlf=r.random() < 0.2, lower=r.random() < 0.2)
+    elif fmt == "2bit":
+        gp = str(tmp_path / ("c%d.2bit" % seed))
+        spec = casoff.synth_spec(seed=seed, n_contigs=r.randint(1, 3), total_len=r.randint(P, 5000), n_frac=0.05,
+                                 n_run_min=3, n_run_max=40, lower_frac=0.3)
+        casoff.synth_write(spec, gp, 1)
+    else:
+        gp = str(tmp_path / ("d%d" % seed))
+        os.makedirs(gp)
+        H.write_fasta(os.path.join(gp, "a.fa"), recs[: max(1, len(recs) // 2)])
+        H.write_fasta(os.path.join(gp, "b.fa"), recs[max(1, len(recs) // 2):] or [("e", "ACGT")])
+        with open(os.path.join(gp, "notes.txt"), "w") as f:
+            f.write("not a genome\n")
+        casoff.synth_write(casoff.synth_spec(seed=seed, n_contigs=2, total_len=2000), os.path.join(gp, "c.2bit"), 1)
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    return gp, text
+
+
+@pytest.mark.parametrize("seed", range(48))
+def test_random_small_cases(ctx, tmp_path, seed):
+    gp, text = random_case(seed, tmp_path)
+    check(ctx, tmp_path, gp, text, "seed %d" % seed)
+
+
+def test_semicolon_and_odd_bytes_in_fasta(ctx, tmp_path):
+    gp = str(tmp_path / "odd.fa")
+    with open(gp, "w") as f:
+        f.write(">x y z\nACGTACGTTTGACCAGGTAC;GGTACCGATTAGGCCAAGG\nAC-GTAC*GTNNNNRYACGTAGGCCAAGGAAGGTT\r\n\n>e\n>w\nacgtAGGtt\n")
+    text = "%s\nNNNNNNNRG 1 1\nACGTACGNN 4\nCCAAGGANN 3 second\n" % gp
+    check(ctx, tmp_path, gp, text, "odd")
+
+
+def test_cli_end_to_end(ctx, tmp_path):
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=5, n_contigs=3, total_len=200000, n_frac=0.01), gp, 0)
+    rr = random.Random(5)
+    guides = ["".join(rr.choice("ACGT") for _ in range(20)) + "NNN" for _ in range(3)]
+    text = "%s\n%s 1 1\n" % (gp, "N" * 21 + "RG") + "".join("%s 4\n" % g for g in guides)
+    ip = str(tmp_path / "in.txt")
+    with open(ip, "w") as f:
+        f.write(text)
+    op = str(tmp_path / "out.txt")
+    sp = str(tmp_path / "sum.txt")
+    p = subprocess.run([casoff.CLI_PATH, ip, "G", op, "--summary", sp], capture_output=True, timeout=300)
+    assert p.returncode == 0, p.stderr
+    exp_text, exp_sum = H.oracle_records(tmp_path, text, name="cli_in.txt", summary=True)
+    with open(op) as f:
+        got = f.read()
+    assert got.split("\n")[:2] == exp_text.split("\n")[:2]
+    assert H.record_lines(got) == H.record_lines(exp_text)
+    with open(sp) as f:
+        assert f.read() == exp_sum
+
+
+def test_batched_multi_query_matches_single(ctx, tmp_path):
+    gp = str(tmp_path / "m.fa")
+    casoff.synth_write(casoff.synth_spec(seed=9, n_contigs=4, total_len=300000, n_frac=0.01), gp, 0)
+    ctx.load(gp)
+    texts = [t for _, t in workloads.group_inputs(workloads.mixed_guides(2, 40, 3, 6, 0.5, 1), gp)]
+    qs = [ctx.query(t) for t in texts]
+    rb = ctx.search(qs, casoff.MODE_BATCHED)
+    for i, t in enumerate(texts):
+        exp = H.record_lines(H.oracle_records(tmp_path, t, name="q%d.txt" % i))
+        assert H.record_lines(rb.format(i)) == exp
