@@ -97,7 +97,7 @@ def lib():
             "cof_format_hits": (i32, [vp, vp, vp, sz, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_free": (None, [vp]),
             "cof_server_start": (i32, [vp, C.POINTER(ServerCfg), C.POINTER(vp)]),
-            "cof_server_submit": (i32, [vp, vp, C.POINTER(u64)]),
+            "cof_server_submit": (i32, [vp, C.POINTER(vp), i32, C.POINTER(u64)]),
             "cof_server_poll": (i32, [vp, u64, i32, C.POINTER(vp), C.POINTER(C.c_double)]),
             "cof_server_stop": (i32, [vp]),
         }
@@ -275,6 +275,43 @@ class Context:
     def __del__(self):
         if getattr(self, "_p", None) is not None and _lib is not None:
             self.close()
+
+
+class Server:
+    """Continuous-batching request loop over one context (cof_server_*)."""
+
+    def __init__(self, ctx, mem_budget_bytes=8 << 30, max_batch_queries=64):
+        self.ctx = ctx
+        cfg = ServerCfg(mem_budget_bytes, max_batch_queries)
+        self._p = C.c_void_p()
+        if lib().cof_server_start(ctx._p, C.byref(cfg), C.byref(self._p)):
+            raise CofError(ctx.error())
+
+    def submit(self, queries):
+        arr = (C.c_void_p * len(queries))(*[q._p.value for q in queries])
+        t = C.c_uint64()
+        if lib().cof_server_submit(self._p, arr, len(queries), C.byref(t)):
+            raise CofError("submit failed")
+        return t.value
+
+    def poll(self, ticket, wait=True):
+        """(Result, latency_ms) when done, None while pending."""
+        out, lat = C.c_void_p(), C.c_double()
+        rc = lib().cof_server_poll(self._p, ticket, int(wait), C.byref(out), C.byref(lat))
+        if rc == 1:
+            return None
+        if rc:
+            raise CofError(self.ctx.error())
+        return Result(self.ctx, out), lat.value
+
+    def stop(self):
+        if self._p.value:
+            lib().cof_server_stop(self._p)
+            self._p = C.c_void_p()
+
+    def __del__(self):
+        if getattr(self, "_p", None) is not None and _lib is not None:
+            self.stop()
 
 
 def record_lines(text):

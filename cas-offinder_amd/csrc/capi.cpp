@@ -35,9 +35,12 @@ int cof_open(int device, cof_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return COF_E_DEVICE;
     auto *c = new cof_ctx();
     c->device = device;
-    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return COF_E_DEVICE; }
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) { delete c; return COF_E_DEVICE; }
+    if (c->lane.init() != hipSuccess) {
+        c->lane.release();
+        delete c;
+        g_noctx_err = "cannot create a HIP stream";
+        return COF_E_DEVICE;
+    }
     *out = c;
     return COF_OK;
 }
@@ -49,13 +52,8 @@ int cof_close(cof_ctx *c) {
         return COF_OK;
     }
     (void)hipSetDevice(c->device);
-    (void)hipStreamSynchronize(c->stream);
+    c->lane.release();
     c->d_lo.release(); c->d_hi.release(); c->d_ot.release(); c->d_poison.release();
-    c->d_hits.release(); c->d_count.release(); c->d_loci.release(); c->d_flags.release();
-    c->d_keys.release(); c->d_scratch.release();
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    (void)hipStreamDestroy(c->stream);
     delete c;
     return COF_OK;
 }
@@ -184,7 +182,7 @@ int cof_search(cof_ctx *c, cof_query *const *queries, int n_queries, int mode, c
 
 int cof_stats_get(const cof_ctx *c, cof_stats *out) {
     if (!c || !out) return COF_E_INVALID;
-    *out = c->stats;
+    *out = c->lane.stats;
     return COF_OK;
 }
 

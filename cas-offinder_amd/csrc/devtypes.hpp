@@ -27,11 +27,14 @@ struct PatternDesc {
     uint8_t rk[kMaxPattern], rm[kMaxPattern];
 };
 
-// Compare key for the kernels: for each class, the key positions where a base
-// of that class counts as a mismatch, as bit b <-> key position cbase + b,
-// `NW` 32-bit words per class.  Stored as 8 dwords (NW = 1) or 16 dwords
-// (NW = 2): m[class][word] at class * NW + word, threshold at 5 * NW.
-constexpr int key_stride(int nw) { return nw == 1 ? 8 : 16; }
+// Compare-key records (bit b of a word <-> key position cbase + 32*word + b):
+//  concrete (every comparable key base is A/C/G/T): klo[NW], khi[NW],
+//    care[NW], bias; 4 dwords (NW = 1) or 8 (NW = 2);
+//  general (IUPAC codes): for each class T, C, A, G, other the positions where
+//    that class mismatches, [class * NW + word], then bias; 8 or 12 dwords.
+//  bias = -(threshold + 1) as uint32: popcount(mismatches) + bias < 0 is a hit.
+constexpr int key_stride(int nw, bool gen) { return gen ? (nw == 1 ? 8 : 12) : (nw == 1 ? 4 : 8); }
+constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 3) * nw; }
 
 // Device hit record, identical to cof_hit of casoff.h.
 struct DevHit {

@@ -1,4 +1,4 @@
-// Device context and the two search paths.
+// Device context, work lanes and the two search paths.
 #include "engine.hpp"
 
 #include <algorithm>
@@ -15,13 +15,204 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
     return (double)ms;
 }
 
-#define HIP_TRY(expr, what)                          \
-    do {                                             \
-        hipError_t e_ = (expr);                      \
-        if (e_ != hipSuccess) return hip_fail(e_, what); \
+#define HIP_TRY(expr, what)                                      \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) {                                  \
+            err = std::string(what) + ": " + hipGetErrorString(e_); \
+            return COF_E_DEVICE;                                 \
+        }                                                        \
     } while (0)
 
+// Key tables of all queries, concatenated; off[q] = first dword of query q.
+void key_tables(const std::vector<std::shared_ptr<Query>> &qs, std::vector<uint32_t> &kw, std::vector<size_t> &off) {
+    off.resize(qs.size());
+    kw.clear();
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        off[qi] = kw.size();
+        kw.insert(kw.end(), qs[qi]->key_words.begin(), qs[qi]->key_words.end());
+    }
+}
+
+int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
+                 uint64_t &nhits, std::string &err) {
+    double t_find = 0, t_cmp = 0;
+    uint64_t ncand = 0, npairs = 0, nbase = 0, launches = 0;
+    std::vector<uint32_t> kw;
+    std::vector<size_t> off;
+    key_tables(qs, kw, off);
+    HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
+    if (!kw.empty())
+        HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        const Query &q = *qs[qi];
+        const uint64_t limit = dg.total >= q.P ? dg.total - q.P + 1 : 0;
+        const int stride = key_stride(q.nw, q.gen);
+        for (uint64_t lo = 0; lo < limit; lo += slab_positions) {
+            const uint64_t hi = std::min(limit, lo + slab_positions);
+            unsigned long long found[2] = {0, 0};
+            for (;;) {
+                HIP_TRY(hipMemsetAsync(L.d_count.p + 1, 0, 2 * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
+                HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
+                HIP_TRY(launch_finder(dg, q.pat, lo, hi, L.d_loci.p, L.d_flags.p, L.d_count.p + 1, L.d_loci.n, L.stream),
+                        "finder launch");
+                HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
+                HIP_TRY(hipMemcpyAsync(found, L.d_count.p + 1, sizeof found, hipMemcpyDeviceToHost, L.stream),
+                        "hipMemcpyAsync");
+                HIP_TRY(hipStreamSynchronize(L.stream), "finder");
+                t_find += ms_between(L.ev[0], L.ev[1]);
+                launches++;
+                if (found[0] <= L.d_loci.n) break;
+                const uint64_t want = found[0] + found[0] / 8 + 1024;
+                HIP_TRY(L.d_loci.reserve(want), "hipMalloc(candidates)");
+                HIP_TRY(L.d_flags.reserve(want), "hipMalloc(candidates)");
+            }
+            ncand += found[1];
+            npairs += found[1] * q.keys.size();
+            nbase += found[1] * q.care_sum;
+            if (found[0] == 0) continue;
+            HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
+            for (size_t k = 0; k < q.keys.size(); k++) {
+                HIP_TRY(launch_comparer(dg, q.P, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi] + k * (size_t)stride,
+                                        (uint32_t)k, (uint32_t)qi, lo, L.d_loci.p, L.d_flags.p, found[0], L.d_hits.p,
+                                        L.d_count.p, L.d_hits.n, L.stream),
+                        "comparer launch");
+                launches++;
+            }
+            HIP_TRY(hipEventRecord(L.ev[3], L.stream), "hipEventRecord");
+            HIP_TRY(hipStreamSynchronize(L.stream), "comparer");
+            t_cmp += ms_between(L.ev[2], L.ev[3]);
+        }
+    }
+    unsigned long long h = 0;
+    HIP_TRY(hipMemcpyAsync(&h, L.d_count.p, sizeof h, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(count)");
+    HIP_TRY(hipStreamSynchronize(L.stream), "count");
+    nhits = h;
+    L.stats.t_finder_ms = t_find;
+    L.stats.t_comparer_ms = t_cmp;
+    L.stats.t_fused_ms = 0;
+    L.stats.n_candidates = ncand;
+    L.stats.n_pairs = npairs;
+    L.stats.n_base_cmp = nbase;
+    L.stats.n_launches = launches;
+    return COF_OK;
+}
+
+int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs, uint64_t &nhits,
+                   std::string &err) {
+    std::vector<uint32_t> kw;
+    std::vector<size_t> off;
+    key_tables(qs, kw, off);
+    HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
+    if (!kw.empty())
+        HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
+    HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
+    uint64_t launches = 0;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        const Query &q = *qs[qi];
+        if (q.keys.empty()) continue;
+        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], (uint32_t)q.keys.size(),
+                             (uint32_t)qi, L.d_hits.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hits.n, L.stream),
+                "fused launch");
+        launches++;
+    }
+    HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
+    std::vector<unsigned long long> cnt(2 + qs.size());
+    HIP_TRY(hipMemcpyAsync(cnt.data(), L.d_count.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                           L.stream),
+            "hipMemcpyAsync(count)");
+    HIP_TRY(hipStreamSynchronize(L.stream), "fused");
+    L.stats.t_fused_ms = ms_between(L.ev[0], L.ev[1]);
+    L.stats.t_comparer_ms = L.stats.t_fused_ms;
+    L.stats.t_finder_ms = 0;
+    uint64_t ncand = 0, npairs = 0, nbase = 0;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        ncand += cnt[2 + qi];
+        npairs += cnt[2 + qi] * qs[qi]->keys.size();
+        nbase += cnt[2 + qi] * qs[qi]->care_sum;
+    }
+    L.stats.n_candidates = ncand;
+    L.stats.n_pairs = npairs;
+    L.stats.n_base_cmp = nbase;
+    L.stats.n_launches = launches;
+    nhits = cnt[0];
+    return COF_OK;
+}
+
 }  // namespace
+
+hipError_t Lane::init() {
+    hipError_t e = hipStreamCreate(&stream);
+    if (e != hipSuccess) return e;
+    for (auto &x : ev)
+        if ((e = hipEventCreate(&x)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+void Lane::release() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    d_hits.release();
+    d_count.release();
+    d_loci.release();
+    d_flags.release();
+    d_keys.release();
+    for (auto &x : ev)
+        if (x) (void)hipEventDestroy(x), x = nullptr;
+    if (stream) (void)hipStreamDestroy(stream);
+    stream = nullptr;
+}
+
+int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
+                    int mode, cof_result **out, std::string &err) {
+    *out = nullptr;
+    if (qs.size() > 255) { err = "At most 255 queries per search."; return COF_E_INVALID; }
+    if (mode != COF_MODE_PLAIN && mode != COF_MODE_BATCHED) { err = "Unknown search mode."; return COF_E_INVALID; }
+    const auto t0 = std::chrono::steady_clock::now();
+    HIP_TRY(L.d_count.reserve(3 + qs.size()), "hipMalloc(counters)");
+    if (!L.d_hits.p) HIP_TRY(L.d_hits.reserve(1 << 20), "hipMalloc(hits)");
+    uint64_t nhits = 0;
+    for (int attempt = 0; attempt < 4; attempt++) {
+        HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
+        int rc = mode == COF_MODE_PLAIN ? search_plain(L, dg, slab_positions, qs, nhits, err)
+                                        : search_batched(L, dg, qs, nhits, err);
+        if (rc) return rc;
+        if (nhits <= L.d_hits.n) break;
+        HIP_TRY(L.d_hits.reserve(nhits + nhits / 4 + 1024), "hipMalloc(hits)");  // overflow: grow and rerun
+    }
+    if (nhits > L.d_hits.n) { err = "Hit buffer overflow."; return COF_E_NOMEM; }
+    auto *r = new cof_result();
+    r->hits.resize(nhits);
+    static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
+    if (nhits) {
+        hipError_t e = hipMemcpyAsync(r->hits.data(), L.d_hits.p, nhits * sizeof(cof_hit), hipMemcpyDeviceToHost, L.stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
+        if (e != hipSuccess) {
+            delete r;
+            err = std::string("hipMemcpy(hits): ") + hipGetErrorString(e);
+            return COF_E_DEVICE;
+        }
+    }
+    std::sort(r->hits.begin(), r->hits.end(), [](const cof_hit &a, const cof_hit &b) {
+        if (a.query != b.query) return a.query < b.query;
+        if (a.key != b.key) return a.key < b.key;
+        if (a.pos != b.pos) return a.pos < b.pos;
+        return a.strand < b.strand;
+    });
+    r->queries = qs;
+    r->qbegin.assign(qs.size() + 1, 0);
+    size_t h = 0;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        r->qbegin[qi] = h;
+        while (h < r->hits.size() && r->hits[h].query == qi) h++;
+    }
+    r->qbegin[qs.size()] = r->hits.size();
+    const auto t1 = std::chrono::steady_clock::now();
+    L.stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    L.stats.n_hits = nhits;
+    L.stats.genome_len = dg.total;
+    *out = r;
+    return COF_OK;
+}
 
 int cof_ctx::upload_genome() {
     genome_ready = false;
@@ -48,149 +239,8 @@ int cof_ctx::upload_genome() {
     return COF_OK;
 }
 
-int cof_ctx::search_plain(const std::vector<std::shared_ptr<Query>> &qs, uint64_t &nhits) {
-    double t_find = 0, t_cmp = 0;
-    uint64_t ncand = 0, npairs = 0, nbase = 0, launches = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        const Query &q = *qs[qi];
-        const uint64_t limit = dg.total >= q.P ? dg.total - q.P + 1 : 0;
-        const int stride = key_stride(q.nw);
-        for (uint64_t lo = 0; lo < limit; lo += slab_positions) {
-            const uint64_t hi = std::min(limit, lo + slab_positions);
-            unsigned long long found[2] = {0, 0};
-            for (;;) {
-                HIP_TRY(hipMemsetAsync(d_count.p + 1, 0, 2 * sizeof(unsigned long long), stream), "hipMemsetAsync");
-                HIP_TRY(hipEventRecord(ev[0], stream), "hipEventRecord");
-                HIP_TRY(launch_finder(dg, q.pat, lo, hi, d_loci.p, d_flags.p, d_count.p + 1, d_loci.n, stream),
-                        "finder launch");
-                HIP_TRY(hipEventRecord(ev[1], stream), "hipEventRecord");
-                HIP_TRY(hipMemcpyAsync(found, d_count.p + 1, sizeof found, hipMemcpyDeviceToHost, stream),
-                        "hipMemcpyAsync");
-                HIP_TRY(hipStreamSynchronize(stream), "finder");
-                t_find += ms_between(ev[0], ev[1]);
-                launches++;
-                if (found[0] <= d_loci.n) break;
-                const uint64_t want = found[0] + found[0] / 8 + 1024;
-                HIP_TRY(d_loci.reserve(want), "hipMalloc(candidates)");
-                HIP_TRY(d_flags.reserve(want), "hipMalloc(candidates)");
-            }
-            ncand += found[1];
-            npairs += found[1] * q.keys.size();
-            nbase += found[1] * q.care_sum;
-            if (found[0] == 0) continue;
-            HIP_TRY(hipEventRecord(ev[2], stream), "hipEventRecord");
-            for (size_t k = 0; k < q.keys.size(); k++) {
-                HIP_TRY(launch_comparer(dg, q.P, q.nw, (uint32_t)q.cbase, &q.key_words[k * (size_t)stride], (uint32_t)k,
-                                        (uint32_t)qi, lo, d_loci.p, d_flags.p, found[0], d_hits.p, d_count.p,
-                                        d_hits.n, stream),
-                        "comparer launch");
-                launches++;
-            }
-            HIP_TRY(hipEventRecord(ev[3], stream), "hipEventRecord");
-            HIP_TRY(hipStreamSynchronize(stream), "comparer");
-            t_cmp += ms_between(ev[2], ev[3]);
-        }
-    }
-    unsigned long long h = 0;
-    HIP_TRY(hipMemcpy(&h, d_count.p, sizeof h, hipMemcpyDeviceToHost), "hipMemcpy(count)");
-    nhits = h;
-    stats.t_finder_ms = t_find;
-    stats.t_comparer_ms = t_cmp;
-    stats.t_fused_ms = 0;
-    stats.n_candidates = ncand;
-    stats.n_pairs = npairs;
-    stats.n_base_cmp = nbase;
-    stats.n_launches = launches;
-    return COF_OK;
-}
-
-int cof_ctx::search_batched(const std::vector<std::shared_ptr<Query>> &qs, uint64_t &nhits) {
-    std::vector<uint32_t> kw;
-    std::vector<size_t> off(qs.size());
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        off[qi] = kw.size();
-        kw.insert(kw.end(), qs[qi]->key_words.begin(), qs[qi]->key_words.end());
-    }
-    HIP_TRY(d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
-    if (!kw.empty())
-        HIP_TRY(hipMemcpyAsync(d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, stream), "hipMemcpy(keys)");
-    HIP_TRY(hipEventRecord(ev[0], stream), "hipEventRecord");
-    uint64_t launches = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        const Query &q = *qs[qi];
-        if (q.keys.empty()) continue;
-        HIP_TRY(launch_fused(dg, q.pat, q.nw, (uint32_t)q.cbase, d_keys.p + off[qi], (uint32_t)q.keys.size(),
-                             (uint32_t)qi, d_hits.p, d_count.p, d_count.p + 2 + qi, d_hits.n, stream),
-                "fused launch");
-        launches++;
-    }
-    HIP_TRY(hipEventRecord(ev[1], stream), "hipEventRecord");
-    std::vector<unsigned long long> cnt(2 + qs.size());
-    HIP_TRY(hipMemcpyAsync(cnt.data(), d_count.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
-                           stream),
-            "hipMemcpyAsync(count)");
-    HIP_TRY(hipStreamSynchronize(stream), "fused");
-    stats.t_fused_ms = ms_between(ev[0], ev[1]);
-    stats.t_comparer_ms = stats.t_fused_ms;
-    stats.t_finder_ms = 0;
-    uint64_t ncand = 0, npairs = 0, nbase = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        ncand += cnt[2 + qi];
-        npairs += cnt[2 + qi] * qs[qi]->keys.size();
-        nbase += cnt[2 + qi] * qs[qi]->care_sum;
-    }
-    stats.n_candidates = ncand;
-    stats.n_pairs = npairs;
-    stats.n_base_cmp = nbase;
-    stats.n_launches = launches;
-    nhits = cnt[0];
-    return COF_OK;
-}
-
 int cof_ctx::search(const std::vector<std::shared_ptr<Query>> &qs, int mode, cof_result **out) {
     *out = nullptr;
     if (!genome_ready) return fail(COF_E_INVALID, "No genome loaded.");
-    if (qs.size() > 255) return fail(COF_E_INVALID, "At most 255 queries per search.");
-    if (mode != COF_MODE_PLAIN && mode != COF_MODE_BATCHED) return fail(COF_E_INVALID, "Unknown search mode.");
-    const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(d_count.reserve(2 + qs.size()), "hipMalloc(counters)");
-    if (!d_hits.p) HIP_TRY(d_hits.reserve(1 << 20), "hipMalloc(hits)");
-    uint64_t nhits = 0;
-    for (int attempt = 0; attempt < 4; attempt++) {
-        HIP_TRY(hipMemsetAsync(d_count.p, 0, d_count.n * sizeof(unsigned long long), stream), "hipMemsetAsync");
-        int rc = mode == COF_MODE_PLAIN ? search_plain(qs, nhits) : search_batched(qs, nhits);
-        if (rc) return rc;
-        if (nhits <= d_hits.n) break;
-        HIP_TRY(d_hits.reserve(nhits + nhits / 4 + 1024), "hipMalloc(hits)");  // overflow: grow and rerun
-    }
-    if (nhits > d_hits.n) return fail(COF_E_NOMEM, "Hit buffer overflow.");
-    auto *r = new cof_result();
-    r->hits.resize(nhits);
-    static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
-    if (nhits) {
-        hipError_t e = hipMemcpy(r->hits.data(), d_hits.p, nhits * sizeof(cof_hit), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) { delete r; return hip_fail(e, "hipMemcpy(hits)"); }
-    }
-    std::sort(r->hits.begin(), r->hits.end(), [](const cof_hit &a, const cof_hit &b) {
-        if (a.query != b.query) return a.query < b.query;
-        if (a.key != b.key) return a.key < b.key;
-        if (a.pos != b.pos) return a.pos < b.pos;
-        return a.strand < b.strand;
-    });
-    r->queries = qs;
-    r->qbegin.assign(qs.size() + 1, 0);
-    {
-        size_t h = 0;
-        for (size_t qi = 0; qi < qs.size(); qi++) {
-            r->qbegin[qi] = h;
-            while (h < r->hits.size() && r->hits[h].query == qi) h++;
-        }
-        r->qbegin[qs.size()] = r->hits.size();
-    }
-    const auto t1 = std::chrono::steady_clock::now();
-    stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
-    stats.n_hits = nhits;
-    stats.genome_len = dg.total;
-    *out = r;
-    return COF_OK;
+    return run_search(lane, dg, slab_positions, qs, mode, out, err);
 }

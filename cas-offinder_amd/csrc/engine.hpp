@@ -1,9 +1,10 @@
-// Device context: resident genome, work buffers, stream and the two search
-// paths (plain: reference launch model; batched: fused per-query launches).
+// Device context: resident genome, work lanes (stream + buffers) and the two
+// search paths (plain: reference launch model; batched: fused per-query launches).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
 
+#include <algorithm>
 #include <memory>
 #include <string>
 #include <vector>
@@ -27,12 +28,12 @@ namespace cof {
 
 hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
                          uint32_t *loci, uint8_t *flags, unsigned long long *count, uint64_t cap, hipStream_t s);
-hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, uint32_t cbase, const uint32_t *key,
+hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *key,
                            uint32_t key_index, uint32_t qid, uint64_t slab_lo, const uint32_t *loci,
                            const uint8_t *flags, uint64_t n, DevHit *hits, unsigned long long *count, uint64_t cap,
                            hipStream_t s);
-hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, uint32_t cbase, const uint32_t *keys,
-                        uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
+hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
+                        const uint32_t *keys, uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
                         unsigned long long *cand_count, uint64_t cap, hipStream_t s);
 
 template <typename T>
@@ -53,40 +54,43 @@ struct DevBuf {
     }
 };
 
+// One in-order stream with its own work buffers; searches on different lanes
+// may run concurrently over the same resident genome.
+struct Lane {
+    hipStream_t stream = nullptr;
+    DevBuf<DevHit> d_hits;
+    DevBuf<unsigned long long> d_count;  // [0] hits, [1..2] plain finder counters, [2+q] batched candidates
+    DevBuf<uint32_t> d_loci;
+    DevBuf<uint8_t> d_flags;
+    DevBuf<uint32_t> d_keys;
+    hipEvent_t ev[4] = {};
+    cof_stats stats{};
+    hipError_t init();
+    void release();
+};
+
+// Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a
+// COF_* status with the message in err.
+int run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
+               int mode, cof_result **out, std::string &err);
+
 }  // namespace cof
 
 struct cof_ctx {
     int device = 0;
-    hipStream_t stream = nullptr;
     std::string err;
     cof::HostGenome genome;
     cof::DevBuf<uint32_t> d_lo, d_hi, d_ot;
     cof::DevBuf<uint64_t> d_poison;
     cof::DevGenome dg{};
     bool genome_ready = false;
-    // work buffers
-    cof::DevBuf<cof::DevHit> d_hits;
-    cof::DevBuf<unsigned long long> d_count;  // [0] hits, [1] finder candidates, [2..] per-query candidates
-    cof::DevBuf<uint32_t> d_loci;
-    cof::DevBuf<uint8_t> d_flags;
-    cof::DevBuf<uint32_t> d_keys;
-    cof::DevBuf<uint8_t> d_scratch;
-    hipEvent_t ev[8] = {};
-    cof_stats stats{};
+    cof::Lane lane;                        // the context's own lane
     uint64_t slab_positions = 1ull << 31;  // plain path window starts per finder launch
 
     int fail(int code, const std::string &m) {
         err = m;
         return code;
     }
-    int hip_fail(hipError_t e, const char *what) {
-        err = std::string(what) + ": " + hipGetErrorString(e);
-        return COF_E_DEVICE;
-    }
     int upload_genome();
     int search(const std::vector<std::shared_ptr<cof::Query>> &qs, int mode, cof_result **out);
-
-private:
-    int search_plain(const std::vector<std::shared_ptr<cof::Query>> &qs, uint64_t &nhits);
-    int search_batched(const std::vector<std::shared_ptr<cof::Query>> &qs, uint64_t &nhits);
 };

@@ -15,13 +15,15 @@
 //                    compare key of the query per candidate, no candidate
 //                    list in HBM
 //
-// Mismatch counting: a window of NW words per plane is turned into five class
-// planes (T, C, A, G, other); a key is five class masks (the classes that
-// mismatch at each key position), so the count is
-// popcount(OR_c class_c & mask_c).  The '-' strand is done by reverse
-// complementing the window (bit reverse, hi ^= 1) and comparing the forward
-// key, which counts the same mismatches as the reverse-complemented key
-// against the forward text (cas-offinder.cl:123-154).
+// Mismatch counting works on NW 32-bit words of the window planes taken at
+// key position cbase.  A key whose comparable bases are all A/C/G/T is
+// (klo, khi, care): mismatches = popcount(((lo^klo) | (hi^khi) | ot) & care);
+// a key with IUPAC codes is five class masks (the classes that mismatch at
+// each position): popcount(OR_c class_c & mask_c).  The threshold is folded in
+// as a bias -(threshold+1) added by the popcount, so a score < 0 is a hit.
+// The '-' strand reverse complements the window (bit reverse, hi ^= 1) and
+// compares the forward key, which counts the same mismatches as the reverse
+// complemented key against the forward text (cas-offinder.cl:123-154).
 #include <hip/hip_runtime.h>
 
 #include "devtypes.hpp"
@@ -30,27 +32,31 @@ namespace cof {
 
 namespace {
 
+// Key tables are read-only for a launch: reading them through the constant
+// address space lets the compiler use scalar loads (one per wavefront).
+typedef const __attribute__((address_space(4))) uint32_t *kptr_t;
+
 __device__ __forceinline__ uint32_t funnel(uint32_t hw, uint32_t lw, uint32_t sh) {
     return __builtin_amdgcn_alignbit(hw, lw, sh);  // bits [sh, sh+32) of hw:lw
 }
 
 __device__ __forceinline__ uint32_t sel_mask(uint32_t m, uint32_t c) { return ((m >> c) & 1u) ? ~0u : 0u; }
 
-// planes -> class-match bits for a 5-bit class set `m` (o positions have lo = hi = 0)
+// planes -> class-match bits for a 5-bit class set `m` (ot positions have lo = hi = 0)
 __device__ __forceinline__ uint32_t class_match(uint32_t m, uint32_t l, uint32_t h, uint32_t o) {
     const uint32_t T = ~(l | h | o), C = l & ~h, A = h & ~l, G = l & h;
     return (T & sel_mask(m, kClsT)) | (C & sel_mask(m, kClsC)) | (A & sel_mask(m, kClsA)) |
            (G & sel_mask(m, kClsG)) | (o & sel_mask(m, kClsO));
 }
 
-// PAM test of the 32 starts of LDS word i0 (and words i0+1, i0+2 for the
-// pattern reach): bit b of *fwd / *rev = start b passes the forward /
-// reverse-complement pattern.
+// PAM test of the 32 starts of LDS word i0 (words i0+1, i0+2 give the pattern
+// reach): bit b of *fwd / *rev = start b passes the forward / reverse
+// complement pattern.
 __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32_t *s_lo, const uint32_t *s_hi,
                                              const uint32_t *s_ot, int i0, uint32_t *fwd, uint32_t *rev) {
-    uint32_t L0 = s_lo[i0], L1 = s_lo[i0 + 1], L2 = s_lo[i0 + 2];
-    uint32_t H0 = s_hi[i0], H1 = s_hi[i0 + 1], H2 = s_hi[i0 + 2];
-    uint32_t O0 = s_ot[i0], O1 = s_ot[i0 + 1], O2 = s_ot[i0 + 2];
+    const uint32_t L0 = s_lo[i0], L1 = s_lo[i0 + 1], L2 = s_lo[i0 + 2];
+    const uint32_t H0 = s_hi[i0], H1 = s_hi[i0 + 1], H2 = s_hi[i0 + 2];
+    const uint32_t O0 = s_ot[i0], O1 = s_ot[i0 + 1], O2 = s_ot[i0 + 2];
     uint32_t f = ~0u, r = ~0u;
     for (uint32_t i = 0; i < pd.nf; i++) {
         const uint32_t k = pd.fk[i], sh = k & 31;
@@ -72,8 +78,7 @@ __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32
     *rev = r;
 }
 
-// Bits of the starts [s0, s0+32) that are inside [lo, hi): window starts
-// allowed by the genome end and the slab.
+// Bits of the starts [s0, s0+32) inside [lo, hi).
 __device__ __forceinline__ uint32_t range_mask(uint64_t s0, uint64_t lo, uint64_t hi) {
     uint32_t m = ~0u;
     if (s0 + 32 > hi) m = s0 >= hi ? 0u : (uint32_t)((1ull << (hi - s0)) - 1);
@@ -85,8 +90,7 @@ __device__ __forceinline__ uint32_t range_mask(uint64_t s0, uint64_t lo, uint64_
 // position.  t0 / t1: the start range the workgroup covers.
 __device__ __forceinline__ uint32_t poison_mask(const DevGenome &g, uint32_t P, uint64_t t0, uint64_t t1,
                                                 uint64_t s0) {
-    // first poison >= t0 (uniform binary search)
-    uint64_t lo = 0, n = g.npoison;
+    uint64_t lo = 0, n = g.npoison;  // first poison >= t0 (uniform binary search)
     while (n > 0) {
         const uint64_t half = n >> 1;
         if (g.poison[lo + half] < t0) { lo += half + 1; n -= half + 1; }
@@ -100,8 +104,7 @@ __device__ __forceinline__ uint32_t poison_mask(const DevGenome &g, uint32_t P, 
         if (x >= s0 && a < s0 + 32) {
             const uint32_t lb = a > s0 ? (uint32_t)(a - s0) : 0u;
             const uint32_t hb = x < s0 + 31 ? (uint32_t)(x - s0) : 31u;
-            const uint32_t m = (uint32_t)(((2ull << hb) - 1) & ~((1ull << lb) - 1));
-            keep &= ~m;
+            keep &= ~(uint32_t)(((2ull << hb) - 1) & ~((1ull << lb) - 1));
         }
     }
     return keep;
@@ -154,14 +157,12 @@ __device__ __forceinline__ void emit_hit(DevHit *hits, unsigned long long *count
     }
 }
 
-// Window of NW words at key position cbase for one candidate, reverse
-// complemented for strand 1, as class planes.  `bits` is the bit offset of
-// the first word to read in the plane arrays p_*; rd(i) reads word i.
+// NW words of the window planes starting at plane bit `bit` (rd(i) reads word
+// i of the three planes); for strand 1 the words are reverse complemented.
+// Results have lo = hi = 0 wherever ot = 1.
 template <int NW, typename Rd>
-__device__ __forceinline__ void window_classes(Rd rd, int64_t bit, uint32_t strand, uint32_t (&cT)[NW],
-                                               uint32_t (&cC)[NW], uint32_t (&cA)[NW], uint32_t (&cG)[NW],
-                                               uint32_t (&cO)[NW]) {
-    uint32_t l[NW], h[NW], o[NW];
+__device__ __forceinline__ void window_planes(Rd rd, int64_t bit, uint32_t strand, uint32_t (&l)[NW],
+                                              uint32_t (&h)[NW], uint32_t (&o)[NW]) {
 #pragma unroll
     for (int w = 0; w < NW; w++) {
         const int64_t b = bit + 32 * w;
@@ -180,38 +181,99 @@ __device__ __forceinline__ void window_classes(Rd rd, int64_t bit, uint32_t stra
         for (int w = 0; w < NW; w++) {
             l2[w] = __builtin_bitreverse32(l[NW - 1 - w]);
             o2[w] = __builtin_bitreverse32(o[NW - 1 - w]);
-            h2[w] = __builtin_bitreverse32(h[NW - 1 - w]) ^ ~o2[w];  // complement: hi ^= 1 on bases
+            h2[w] = ~__builtin_bitreverse32(h[NW - 1 - w]) & ~o2[w];  // complement: hi ^= 1 on bases
         }
 #pragma unroll
-        for (int w = 0; w < NW; w++) { l[w] = l2[w]; h[w] = h2[w] & ~o2[w]; o[w] = o2[w]; }
-    }
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        cT[w] = ~(l[w] | h[w] | o[w]);
-        cC[w] = l[w] & ~h[w];
-        cA[w] = h[w] & ~l[w];
-        cG[w] = l[w] & h[w];
-        cO[w] = o[w];
+        for (int w = 0; w < NW; w++) { l[w] = l2[w]; h[w] = h2[w]; o[w] = o2[w]; }
     }
 }
 
-// Key tables are read-only for a launch: reading them through the constant
-// address space lets the compiler use scalar loads (one per wavefront).
-typedef const __attribute__((address_space(4))) uint32_t *kptr_t;
+// Key record: GEN = false (every comparable key base is A/C/G/T): per word
+// klo, khi, care, then bias = -(threshold + 1); GEN = true (IUPAC codes):
+// per word the five class masks (T, C, A, G, other), then bias.
+template <int NW, bool GEN>
+struct KeyFmt {
+    static constexpr int bias = key_bias_index(NW, GEN);
+    static constexpr int stride = key_stride(NW, GEN);
+};
 
-template <int NW, typename KP>
-__device__ __forceinline__ uint32_t key_mismatches(KP kw, const uint32_t (&cT)[NW],
-                                                   const uint32_t (&cC)[NW], const uint32_t (&cA)[NW],
-                                                   const uint32_t (&cG)[NW], const uint32_t (&cO)[NW]) {
-    uint32_t mm = 0;
+// v_bitop3_b32 truth tables (bit index = a*4 + b*2 + c)
+constexpr uint32_t kXorOr = 0xBE;  // (a ^ b) | c
+constexpr uint32_t kOrAnd = 0xC8;  // (a | c) & b
+constexpr uint32_t kAndOr = 0xEA;  // (a & b) | c
+
+// Window planes of one candidate in the form the key records need: for
+// concrete keys (lo, hi, ot); for IUPAC keys the five class planes.
+template <int NW, bool GEN>
+struct Win {
+    uint32_t p[GEN ? 5 : 3][NW];
+    __device__ __forceinline__ Win(const uint32_t (&l)[NW], const uint32_t (&h)[NW], const uint32_t (&o)[NW]) {
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            if (GEN) {
+                p[kClsT][w] = ~(l[w] | h[w] | o[w]);
+                p[kClsC][w] = l[w] & ~h[w];
+                p[kClsA][w] = h[w] & ~l[w];
+                p[kClsG][w] = l[w] & h[w];
+                p[kClsO][w] = o[w];
+            } else {
+                p[0][w] = l[w];
+                p[1][w] = h[w];
+                p[2][w] = o[w];
+            }
+        }
+    }
+};
+
+// popcount(mismatches) + bias: negative iff mismatches <= threshold
+template <int NW, bool GEN>
+__device__ __forceinline__ int32_t key_score(kptr_t kw, const Win<NW, GEN> &win) {
+    uint32_t acc = kw[KeyFmt<NW, GEN>::bias];
 #pragma unroll
     for (int w = 0; w < NW; w++) {
-        const uint32_t x = (cT[w] & kw[kClsT * NW + w]) | (cC[w] & kw[kClsC * NW + w]) |
-                           (cA[w] & kw[kClsA * NW + w]) | (cG[w] & kw[kClsG * NW + w]) |
-                           (cO[w] & kw[kClsO * NW + w]);
-        mm += __popc(x);
+        uint32_t x;
+        if (GEN) {
+            x = win.p[kClsO][w] & kw[kClsO * NW + w];
+            x = __builtin_amdgcn_bitop3_b32(win.p[kClsG][w], kw[kClsG * NW + w], x, kAndOr);
+            x = __builtin_amdgcn_bitop3_b32(win.p[kClsA][w], kw[kClsA * NW + w], x, kAndOr);
+            x = __builtin_amdgcn_bitop3_b32(win.p[kClsC][w], kw[kClsC * NW + w], x, kAndOr);
+            x = __builtin_amdgcn_bitop3_b32(win.p[kClsT][w], kw[kClsT * NW + w], x, kAndOr);
+        } else {
+            const uint32_t hx = win.p[1][w] ^ kw[NW + w];
+            x = __builtin_amdgcn_bitop3_b32(win.p[0][w], kw[w], hx, kXorOr);        // (lo^klo) | (hi^khi)
+            x = __builtin_amdgcn_bitop3_b32(x, kw[2 * NW + w], win.p[2][w], kOrAnd);  // (x | ot) & care
+        }
+        acc += (uint32_t)__popc(x);
     }
-    return mm;
+    return (int32_t)acc;
+}
+
+// Scores the keys [0, nkeys) of table `kp` against one window and emits hits;
+// four keys per branch on the combined sign.
+template <int NW, bool GEN>
+__device__ __forceinline__ void score_keys(kptr_t kp, uint32_t nkeys, const uint32_t (&l)[NW],
+                                           const uint32_t (&h)[NW], const uint32_t (&o)[NW], uint64_t pos,
+                                           uint32_t strand, uint32_t qid, uint32_t key0, DevHit *hits,
+                                           unsigned long long *count, uint64_t cap) {
+    constexpr int S = KeyFmt<NW, GEN>::stride, B = KeyFmt<NW, GEN>::bias;
+    const Win<NW, GEN> win(l, h, o);
+    uint32_t k = 0;
+    for (; k + 4 <= nkeys; k += 4, kp += 4 * S) {
+        const int32_t d0 = key_score<NW, GEN>(kp, win);
+        const int32_t d1 = key_score<NW, GEN>(kp + S, win);
+        const int32_t d2 = key_score<NW, GEN>(kp + 2 * S, win);
+        const int32_t d3 = key_score<NW, GEN>(kp + 3 * S, win);
+        if ((d0 | d1 | d2 | d3) < 0) {
+            if (d0 < 0) emit_hit(hits, count, cap, pos, key0 + k, (uint32_t)(d0 - (int32_t)kp[B]), strand, qid);
+            if (d1 < 0) emit_hit(hits, count, cap, pos, key0 + k + 1, (uint32_t)(d1 - (int32_t)kp[S + B]), strand, qid);
+            if (d2 < 0) emit_hit(hits, count, cap, pos, key0 + k + 2, (uint32_t)(d2 - (int32_t)kp[2 * S + B]), strand, qid);
+            if (d3 < 0) emit_hit(hits, count, cap, pos, key0 + k + 3, (uint32_t)(d3 - (int32_t)kp[3 * S + B]), strand, qid);
+        }
+    }
+    for (; k < nkeys; k++, kp += S) {
+        const int32_t d = key_score<NW, GEN>(kp, win);
+        if (d < 0) emit_hit(hits, count, cap, pos, key0 + k, (uint32_t)(d - (int32_t)kp[B]), strand, qid);
+    }
 }
 
 }  // namespace
@@ -268,9 +330,8 @@ __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
 
 struct ComparerArgs {
     DevGenome g;
-    uint32_t P, cbase;
-    uint32_t key[16];  // one compare key, key_stride(NW) dwords
-    uint32_t key_index, qid;
+    uint32_t P, cbase, key_index, qid;
+    const uint32_t *key;  // one key record
     uint64_t slab_lo;
     const uint32_t *loci;
     const uint8_t *flags;
@@ -280,7 +341,7 @@ struct ComparerArgs {
     uint64_t cap;
 };
 
-template <int NW>
+template <int NW, bool GEN>
 __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
     const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= a.n) return;
@@ -293,16 +354,13 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
         h = in ? g.hi[wi] : 0u;
         o = in ? g.ot[wi] : ~0u;
     };
-    const uint32_t thr = a.key[5 * NW];
-#pragma unroll
     for (uint32_t strand = 0; strand < 2; strand++) {
         if (strand == 0 ? !(fl == 0 || fl == 1) : !(fl == 0 || fl == 2)) continue;
         const int64_t bit = strand == 0 ? (int64_t)pos + a.cbase
                                         : (int64_t)pos + (int64_t)a.P - 32 * NW - (int64_t)a.cbase;
-        uint32_t cT[NW], cC[NW], cA[NW], cG[NW], cO[NW];
-        window_classes<NW>(rd, bit, strand, cT, cC, cA, cG, cO);
-        const uint32_t mm = key_mismatches<NW>(a.key, cT, cC, cA, cG, cO);
-        if (mm <= thr) emit_hit(a.hits, a.count, a.cap, pos, a.key_index, mm, strand, a.qid);
+        uint32_t l[NW], h[NW], o[NW];
+        window_planes<NW>(rd, bit, strand, l, h, o);
+        score_keys<NW, GEN>((kptr_t)a.key, 1, l, h, o, pos, strand, a.qid, a.key_index, a.hits, a.count, a.cap);
     }
 }
 
@@ -312,7 +370,7 @@ struct FusedArgs {
     DevGenome g;
     PatternDesc pd;
     uint32_t cbase, nkeys, qid, pad;
-    const uint32_t *keys;  // nkeys * key_stride(NW) dwords
+    const uint32_t *keys;  // nkeys key records
     uint64_t limit;        // valid window starts are [0, limit)
     DevHit *hits;
     unsigned long long *count;
@@ -320,10 +378,12 @@ struct FusedArgs {
     uint64_t cap;
 };
 
-template <int NW>
+constexpr int kListCap = 4096;  // LDS candidate entries per pass
+
+template <int NW, bool GEN>
 __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
-    __shared__ uint16_t s_list[2 * kTilePos];
+    __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
     const int t = threadIdx.x;
     const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
@@ -338,23 +398,9 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
     fwd &= keep;
     rev &= keep;
     uint32_t n;
-    uint32_t off = block_scan(__popc(fwd) + __popc(rev), s_wsum, &n);
-    {
-        uint32_t m = fwd;
-        while (m) {
-            const uint32_t b = __builtin_ctz(m);
-            m &= m - 1;
-            s_list[off++] = (uint16_t)((t * 32 + b) << 1);
-        }
-        m = rev;
-        while (m) {
-            const uint32_t b = __builtin_ctz(m);
-            m &= m - 1;
-            s_list[off++] = (uint16_t)(((t * 32 + b) << 1) | 1);
-        }
-    }
+    const uint32_t mine = __popc(fwd) + __popc(rev);
+    const uint32_t first = block_scan(mine, s_wsum, &n);
     if (t == 0 && n) atomicAdd(a.cand_count, (unsigned long long)n);
-    __syncthreads();
 
     const int lane = t & 63, wave = t >> 6;
     auto rd = [&](int64_t wi, uint32_t &l, uint32_t &h, uint32_t &o) {
@@ -362,24 +408,40 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
         h = s_hi[wi];
         o = s_ot[wi];
     };
-    const uint32_t stride = key_stride(NW);
-    for (uint32_t base = (uint32_t)wave * 64; base < n; base += 256) {
-        const uint32_t idx = base + lane;
-        if (idx < n) {
-            const uint32_t e = s_list[idx];
-            const uint32_t ls = e >> 1, strand = e & 1;
-            const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
-                                            : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
-            uint32_t cT[NW], cC[NW], cA[NW], cG[NW], cO[NW];
-            window_classes<NW>(rd, bit, strand, cT, cC, cA, cG, cO);
-            const uint64_t pos = t0 + ls;
-            kptr_t kp = (kptr_t)a.keys;
-#pragma unroll 4
-            for (uint32_t k = 0; k < a.nkeys; k++, kp += stride) {
-                const uint32_t mm = key_mismatches<NW>(kp, cT, cC, cA, cG, cO);
-                if (mm <= kp[5 * NW]) emit_hit(a.hits, a.count, a.cap, pos, k, mm, strand, a.qid);
+    for (uint32_t pass = 0; pass < n; pass += kListCap) {
+        const uint32_t cnt = min(n - pass, (uint32_t)kListCap);
+        if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
+            uint32_t idx = first;
+            uint32_t m = fwd;
+            while (m) {
+                const uint32_t b = __builtin_ctz(m);
+                m &= m - 1;
+                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)((t * 32 + b) << 1);
+                idx++;
+            }
+            m = rev;
+            while (m) {
+                const uint32_t b = __builtin_ctz(m);
+                m &= m - 1;
+                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)(((t * 32 + b) << 1) | 1);
+                idx++;
             }
         }
+        __syncthreads();
+        for (uint32_t base = (uint32_t)wave * 64; base < cnt; base += 256) {
+            const uint32_t i = base + lane;
+            if (i < cnt) {
+                const uint32_t e = s_list[i];
+                const uint32_t ls = e >> 1, strand = e & 1;
+                const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
+                                                : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
+                uint32_t l[NW], h[NW], o[NW];
+                window_planes<NW>(rd, bit, strand, l, h, o);
+                score_keys<NW, GEN>((kptr_t)a.keys, a.nkeys, l, h, o, t0 + ls, strand, a.qid, 0, a.hits, a.count,
+                                    a.cap);
+            }
+        }
+        __syncthreads();
     }
 }
 
@@ -395,33 +457,25 @@ hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t sla
     return hipGetLastError();
 }
 
-hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, uint32_t cbase, const uint32_t *key,
+hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *key,
                            uint32_t key_index, uint32_t qid, uint64_t slab_lo, const uint32_t *loci,
                            const uint8_t *flags, uint64_t n, DevHit *hits, unsigned long long *count, uint64_t cap,
                            hipStream_t s) {
     if (n == 0) return hipSuccess;
-    ComparerArgs a{};
-    a.g = g;
-    a.P = P;
-    a.cbase = cbase;
-    for (int i = 0; i < key_stride(nw); i++) a.key[i] = key[i];
-    a.key_index = key_index;
-    a.qid = qid;
-    a.slab_lo = slab_lo;
-    a.loci = loci;
-    a.flags = flags;
-    a.n = n;
-    a.hits = hits;
-    a.count = count;
-    a.cap = cap;
-    const uint64_t blocks = (n + 255) / 256;
-    if (nw == 1) hipLaunchKernelGGL(comparer_kernel<1>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(comparer_kernel<2>, dim3((uint32_t)blocks), dim3(256), 0, s, a);
+    ComparerArgs a{g, P, cbase, key_index, qid, key, slab_lo, loci, flags, n, hits, count, cap};
+    const dim3 grid((uint32_t)((n + 255) / 256)), block(256);
+    if (nw == 1) {
+        if (gen) hipLaunchKernelGGL((comparer_kernel<1, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((comparer_kernel<1, false>), grid, block, 0, s, a);
+    } else {
+        if (gen) hipLaunchKernelGGL((comparer_kernel<2, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((comparer_kernel<2, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 
-hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, uint32_t cbase, const uint32_t *keys,
-                        uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
+hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
+                        const uint32_t *keys, uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
                         unsigned long long *cand_count, uint64_t cap, hipStream_t s) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0) return hipSuccess;
@@ -437,9 +491,14 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, uint3
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
-    const uint64_t tiles = (limit + kTilePos - 1) / kTilePos;
-    if (nw == 1) hipLaunchKernelGGL(fused_kernel<1>, dim3((uint32_t)tiles), dim3(256), 0, s, a);
-    else hipLaunchKernelGGL(fused_kernel<2>, dim3((uint32_t)tiles), dim3(256), 0, s, a);
+    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos)), block(256);
+    if (nw == 1) {
+        if (gen) hipLaunchKernelGGL((fused_kernel<1, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fused_kernel<1, false>), grid, block, 0, s, a);
+    } else {
+        if (gen) hipLaunchKernelGGL((fused_kernel<2, true>), grid, block, 0, s, a);
+        else hipLaunchKernelGGL((fused_kernel<2, false>), grid, block, 0, s, a);
+    }
     return hipGetLastError();
 }
 

@@ -243,7 +243,13 @@ int encode_query(Query &q, std::string &err) {
     q.cbase = lo;
     const int span = hi - lo + 1;
     q.nw = span <= 32 ? 1 : 2;
-    const int stride = key_stride(q.nw);
+    q.gen = false;
+    for (auto &k : q.keys)
+        for (uint32_t i = 0; i < P; i++) {
+            const char c = k.seq[i];
+            if (mismatch_classes(c) && c != 'A' && c != 'C' && c != 'G' && c != 'T') q.gen = true;
+        }
+    const int nw = q.nw, stride = key_stride(nw, q.gen);
     q.key_words.assign(q.keys.size() * (size_t)stride, 0);
     q.care_sum = 0;
     for (size_t n = 0; n < q.keys.size(); n++) {
@@ -254,10 +260,19 @@ int encode_query(Query &q, std::string &err) {
             if (!mm) continue;
             q.care_sum++;
             const int b = (int)i - q.cbase;
-            for (int c = 0; c < 5; c++)
-                if (mm & (1u << c)) w[c * q.nw + (b >> 5)] |= 1u << (b & 31);
+            const uint32_t bit = 1u << (b & 31);
+            if (q.gen) {
+                for (int c = 0; c < 5; c++)
+                    if (mm & (1u << c)) w[c * nw + (b >> 5)] |= bit;
+            } else {
+                // 2-bit code of the key base: T=0, C=1, A=2, G=3
+                const uint32_t code = s[i] == 'T' ? 0 : s[i] == 'C' ? 1 : s[i] == 'A' ? 2 : 3;
+                if (code & 1) w[(b >> 5)] |= bit;
+                if (code & 2) w[nw + (b >> 5)] |= bit;
+                w[2 * nw + (b >> 5)] |= bit;
+            }
         }
-        w[5 * q.nw] = q.keys[n].threshold;
+        w[key_bias_index(nw, q.gen)] = (uint32_t)(-(int64_t)q.keys[n].threshold - 1);
     }
     return 0;
 }

@@ -32,8 +32,9 @@ struct Query {
     // device encodings (encode_query)
     PatternDesc pat{};
     int nw = 1;
+    bool gen = false;  // some key has an IUPAC code at a comparable position
     int cbase = 0;
-    std::vector<uint32_t> key_words;  // keys.size() * key_stride(nw)
+    std::vector<uint32_t> key_words;  // keys.size() * key_stride(nw, gen)
     uint64_t guides = 0;              // guide lines read
     uint64_t care_sum = 0;            // sum over keys of positions that can mismatch
 };

@@ -159,9 +159,13 @@ typedef struct {
     uint32_t max_batch_queries;  /* queries merged into one pass */
 } cof_server_cfg;
 int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out);
-/* Takes ownership of q.  *ticket identifies the request. */
-int cof_server_submit(cof_server *s, cof_query *q, uint64_t *ticket);
-/* 0: done (*out set, latency in *latency_ms), 1: pending, <0 error. */
+/* One request of n_queries queries (for example one per pattern group); the
+ * server keeps its own references, the caller may free the queries.
+ * *ticket identifies the request. */
+int cof_server_submit(cof_server *s, cof_query *const *queries, int n_queries, uint64_t *ticket);
+/* 0: done (*out set, one result per request with its queries indexed 0..n-1,
+ * latency from submit to records in host memory in *latency_ms),
+ * 1: pending, < 0: error.  A done or failed ticket is forgotten. */
 int cof_server_poll(cof_server *s, uint64_t ticket, int wait, cof_result **out, double *latency_ms);
 int cof_server_stop(cof_server *s);
 
