@@ -26,7 +26,11 @@ import workloads  # noqa: E402
 METRIC = "guide×Gbp/s bit-exact, batched vs plain on 1 GPU; serving p50/p99 latency"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
-OPS_PER_BASE_CMP = 0.75  # cost model of a 4-bit one-hot compare: 6 VALU ops per 8 bases
+# Comparer roofline unit: one (candidate, compare key, 32-position word) costs
+# 3 int32 VALU ops in the bit-sliced compare (2x v_bitop3 + v_bcnt; 6 for keys
+# with IUPAC codes) -- counted by the library as n_model_ops.  The survey's
+# 4-bit one-hot cost model (0.75 ops per base comparison) is reported beside it.
+SURVEY_OPS_PER_BASE_CMP = 0.75
 
 
 def log(*a):
@@ -48,6 +52,20 @@ def build_workload(name, ctx, n_guides=None):
     qs = [ctx.query(t) for t in texts]
     guides = sum(len([l for l in t.split("\n")[2:] if l.strip()]) for t in texts)
     return texts, qs, guides, desc
+
+
+def calibrate_copy(nbytes):
+    """A device copy of known size (torch elementwise kernel) for scaling the
+    FETCH_SIZE / WRITE_SIZE counters."""
+    import torch
+    n = nbytes // 4
+    x = torch.ones(n, dtype=torch.int32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    for _ in range(3):
+        torch.add(x, 1, out=y)
+    torch.cuda.synchronize()
+    print(json.dumps({"calibrate_copy_bytes": n * 4, "launches": 3}), flush=True)
 
 
 def time_steps(ctx, qs, mode, steps, warmup):
@@ -105,7 +123,13 @@ def main():
     ap.add_argument("--plain-steps", type=int, default=2)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--genome-len", type=int, default=3_100_000_000)
+    ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
+    ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
+                    help="only run a device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy(args.calibrate_copy)
+        return
     if args.gpus != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding).",
               file=sys.stderr)
@@ -119,21 +143,26 @@ def main():
     log("genome: %d positions, %d contigs, generated+uploaded in %.1f s" % (total, ncontigs, t_load))
     texts, qs, n_guides, desc = build_workload(args.workload, ctx, args.guides)
 
+    log("workload: %s (%d queries, %d guides)" % (args.workload, len(qs), n_guides))
     s_b, k_b, l_b, st_b, r_b = time_steps(ctx, qs, casoff.MODE_BATCHED, args.steps, args.warmup)
     hits_b = r_b.records()
     log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
-    s_p, k_p, l_p, st_p, r_p = time_steps(ctx, qs, casoff.MODE_PLAIN, args.plain_steps, 0)
-    hits_p = r_p.records()
-    log("plain: %.2f ms/step (kernels %.2f ms), %d hits" % (s_p * 1e3, k_p, len(hits_p)))
-    same = len(hits_b) == len(hits_p) and (hits_b == hits_p).all()
+    if args.no_plain:
+        s_p, k_p, l_p, same = float("nan"), float("nan"), 0, None
+    else:
+        s_p, k_p, l_p, st_p, r_p = time_steps(ctx, qs, casoff.MODE_PLAIN, args.plain_steps, 0)
+        hits_p = r_p.records()
+        log("plain: %.2f ms/step (kernels %.2f ms), %d hits" % (s_p * 1e3, k_p, len(hits_p)))
+        same = bool(len(hits_b) == len(hits_p) and (hits_b == hits_p).all())
 
     gbp = total / 1e9
     value = n_guides * gbp / s_b
-    plain_value = n_guides * gbp / s_p
+    plain_value = n_guides * gbp / s_p if not args.no_plain else float("nan")
     avg_launch_ms = k_b / max(l_b, 1)
-    base_cmp = st_b["n_base_cmp"] / max(l_b, 1)
-    achieved = OPS_PER_BASE_CMP * base_cmp / (avg_launch_ms * 1e-3) / 1e12
-    hbm_achieved = 0.375 * total / (avg_launch_ms * 1e-3) / 1e9
+    launch_s = avg_launch_ms * 1e-3
+    achieved = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
+    survey = SURVEY_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
+    hbm_achieved = 0.375 * total / launch_s / 1e9
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -149,18 +178,23 @@ def main():
         "data": "synthetic",
         "config": {"workload": args.workload + ": " + desc, "guides": n_guides, "genome_bp": total,
                    "mode": "batched (fused finder+comparer, one launch per query)"},
-        "plain": {"value": round(plain_value, 2), "ms_per_step": round(s_p * 1e3, 3), "steps": args.plain_steps,
-                  "launches_per_step": l_p, "bit_exact_vs_batched": bool(same)},
-        "batched_over_plain": round(value / plain_value, 2),
+        "plain": None if args.no_plain else {
+            "value": round(plain_value, 2), "ms_per_step": round(s_p * 1e3, 3), "steps": args.plain_steps,
+            "launches_per_step": l_p, "bit_exact_vs_batched": same},
+        "batched_over_plain": None if args.no_plain else round(value / plain_value, 2),
         "candidate_sites_per_s": st_b["n_pairs"] / s_b,
         "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),
                      "base_comparisons": st_b["n_base_cmp"]},
         "genome_generate_upload_s": round(t_load, 2),
         "roofline": {"bound": "valu", "kernel": "fused_kernel", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "avg_launch_ms": round(avg_launch_ms, 4), "traffic": None,
-                     "hbm_equiv": {"achieved": round(hbm_achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                   "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
+                     "traffic": None, "avg_launch_ms": round(avg_launch_ms, 4),
+                     "work_per_launch": "%d int32 ops = 3 per (candidate, key, 32-position word)" % (
+                         st_b["n_model_ops"] / max(l_b, 1)),
+                     "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
+                                           "frac": round(survey / VALU_PEAK_TOPS, 3)},
+                     "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
     }
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, texts)

@@ -47,6 +47,7 @@ class Stats(C.Structure):
         ("n_launches", C.c_uint64),
         ("genome_len", C.c_uint64),
         ("n_base_cmp", C.c_uint64),
+        ("n_model_ops", C.c_uint64),
     ]
 
     def as_dict(self):

@@ -27,14 +27,18 @@ struct PatternDesc {
     uint8_t rk[kMaxPattern], rm[kMaxPattern];
 };
 
-// Compare-key records (bit b of a word <-> key position cbase + 32*word + b):
-//  concrete (every comparable key base is A/C/G/T): klo[NW], khi[NW],
-//    care[NW], bias; 4 dwords (NW = 1) or 8 (NW = 2);
-//  general (IUPAC codes): for each class T, C, A, G, other the positions where
-//    that class mismatches, [class * NW + word], then bias; 8 or 12 dwords.
+// Compare-key tables (bit b of a word <-> key position cbase + 32*word + b).
+//  Concrete (every comparable key base is A/C/G/T): care classes, each a
+//    kClassHdr-dword header [care[NW], nkeys, 0..] followed by nkeys records
+//    [klo[NW], khi[NW], bias, key index] with klo/khi masked by care;
+//    records are 4 dwords (NW = 1) or 8 (NW = 2).
+//  General (IUPAC codes): records [mask[class * NW + word] for the classes
+//    T, C, A, G, other (the positions where that class mismatches), bias,
+//    key index]; 8 or 12 dwords.
 //  bias = -(threshold + 1) as uint32: popcount(mismatches) + bias < 0 is a hit.
+constexpr int kClassHdr = 4;
 constexpr int key_stride(int nw, bool gen) { return gen ? (nw == 1 ? 8 : 12) : (nw == 1 ? 4 : 8); }
-constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 3) * nw; }
+constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 
 // Device hit record, identical to cof_hit of casoff.h.
 struct DevHit {

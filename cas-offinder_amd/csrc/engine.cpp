@@ -25,29 +25,32 @@ double ms_between(hipEvent_t a, hipEvent_t b) {
     } while (0)
 
 // Key tables of all queries, concatenated; off[q] = first dword of query q.
-void key_tables(const std::vector<std::shared_ptr<Query>> &qs, std::vector<uint32_t> &kw, std::vector<size_t> &off) {
+// `single`: the per-key tables of the plain comparer instead of the fused table.
+void key_tables(const std::vector<std::shared_ptr<Query>> &qs, std::vector<uint32_t> &kw, std::vector<size_t> &off,
+                bool single) {
     off.resize(qs.size());
     kw.clear();
     for (size_t qi = 0; qi < qs.size(); qi++) {
         off[qi] = kw.size();
-        kw.insert(kw.end(), qs[qi]->key_words.begin(), qs[qi]->key_words.end());
+        const auto &t = single ? qs[qi]->single_words : qs[qi]->key_words;
+        kw.insert(kw.end(), t.begin(), t.end());
     }
 }
 
 int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
                  uint64_t &nhits, std::string &err) {
     double t_find = 0, t_cmp = 0;
-    uint64_t ncand = 0, npairs = 0, nbase = 0, launches = 0;
+    uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0, launches = 0;
     std::vector<uint32_t> kw;
     std::vector<size_t> off;
-    key_tables(qs, kw, off);
+    key_tables(qs, kw, off, true);
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     for (size_t qi = 0; qi < qs.size(); qi++) {
         const Query &q = *qs[qi];
         const uint64_t limit = dg.total >= q.P ? dg.total - q.P + 1 : 0;
-        const int stride = key_stride(q.nw, q.gen);
+        const int stride = q.single_stride;
         for (uint64_t lo = 0; lo < limit; lo += slab_positions) {
             const uint64_t hi = std::min(limit, lo + slab_positions);
             unsigned long long found[2] = {0, 0};
@@ -70,12 +73,13 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
             ncand += found[1];
             npairs += found[1] * q.keys.size();
             nbase += found[1] * q.care_sum;
+            nmodel += found[1] * q.keys.size() * (uint64_t)q.nw * (q.gen ? 6 : 3);
             if (found[0] == 0) continue;
             HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
             for (size_t k = 0; k < q.keys.size(); k++) {
                 HIP_TRY(launch_comparer(dg, q.P, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi] + k * (size_t)stride,
-                                        (uint32_t)k, (uint32_t)qi, lo, L.d_loci.p, L.d_flags.p, found[0], L.d_hits.p,
-                                        L.d_count.p, L.d_hits.n, L.stream),
+                                        (uint32_t)qi, lo, L.d_loci.p, L.d_flags.p, found[0], L.d_hits.p, L.d_count.p,
+                                        L.d_hits.n, L.stream),
                         "comparer launch");
                 launches++;
             }
@@ -94,6 +98,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     L.stats.n_candidates = ncand;
     L.stats.n_pairs = npairs;
     L.stats.n_base_cmp = nbase;
+    L.stats.n_model_ops = nmodel;
     L.stats.n_launches = launches;
     return COF_OK;
 }
@@ -102,7 +107,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                    std::string &err) {
     std::vector<uint32_t> kw;
     std::vector<size_t> off;
-    key_tables(qs, kw, off);
+    key_tables(qs, kw, off, false);
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
@@ -111,8 +116,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     for (size_t qi = 0; qi < qs.size(); qi++) {
         const Query &q = *qs[qi];
         if (q.keys.empty()) continue;
-        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], (uint32_t)q.keys.size(),
-                             (uint32_t)qi, L.d_hits.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hits.n, L.stream),
+        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], q.key_units, (uint32_t)qi,
+                             L.d_hits.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hits.n, L.stream),
                 "fused launch");
         launches++;
     }
@@ -125,15 +130,17 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_fused_ms = ms_between(L.ev[0], L.ev[1]);
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
-    uint64_t ncand = 0, npairs = 0, nbase = 0;
+    uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
     for (size_t qi = 0; qi < qs.size(); qi++) {
         ncand += cnt[2 + qi];
         npairs += cnt[2 + qi] * qs[qi]->keys.size();
         nbase += cnt[2 + qi] * qs[qi]->care_sum;
+        nmodel += cnt[2 + qi] * qs[qi]->keys.size() * (uint64_t)qs[qi]->nw * (qs[qi]->gen ? 6 : 3);
     }
     L.stats.n_candidates = ncand;
     L.stats.n_pairs = npairs;
     L.stats.n_base_cmp = nbase;
+    L.stats.n_model_ops = nmodel;
     L.stats.n_launches = launches;
     nhits = cnt[0];
     return COF_OK;

@@ -28,12 +28,11 @@ namespace cof {
 
 hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
                          uint32_t *loci, uint8_t *flags, unsigned long long *count, uint64_t cap, hipStream_t s);
-hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *key,
-                           uint32_t key_index, uint32_t qid, uint64_t slab_lo, const uint32_t *loci,
-                           const uint8_t *flags, uint64_t n, DevHit *hits, unsigned long long *count, uint64_t cap,
-                           hipStream_t s);
+hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
+                           uint32_t qid, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
+                           DevHit *hits, unsigned long long *count, uint64_t cap, hipStream_t s);
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
-                        const uint32_t *keys, uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
+                        const uint32_t *table, uint32_t nunits, uint32_t qid, DevHit *hits, unsigned long long *count,
                         unsigned long long *cand_count, uint64_t cap, hipStream_t s);
 
 template <typename T>

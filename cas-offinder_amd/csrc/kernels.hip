@@ -188,91 +188,115 @@ __device__ __forceinline__ void window_planes(Rd rd, int64_t bit, uint32_t stran
     }
 }
 
-// Key record: GEN = false (every comparable key base is A/C/G/T): per word
-// klo, khi, care, then bias = -(threshold + 1); GEN = true (IUPAC codes):
-// per word the five class masks (T, C, A, G, other), then bias.
-template <int NW, bool GEN>
-struct KeyFmt {
-    static constexpr int bias = key_bias_index(NW, GEN);
-    static constexpr int stride = key_stride(NW, GEN);
-};
-
 // v_bitop3_b32 truth tables (bit index = a*4 + b*2 + c)
 constexpr uint32_t kXorOr = 0xBE;  // (a ^ b) | c
-constexpr uint32_t kOrAnd = 0xC8;  // (a | c) & b
 constexpr uint32_t kAndOr = 0xEA;  // (a & b) | c
 
-// Window planes of one candidate in the form the key records need: for
-// concrete keys (lo, hi, ot); for IUPAC keys the five class planes.
+// Concrete key table (every comparable key base is A/C/G/T): a run of care
+// classes, each a header [care[NW], nkeys, 0...] of kClassHdr dwords followed
+// by nkeys records [klo[NW], khi[NW], bias, key index] (klo/khi masked by
+// care).  With the window masked once per class (lc = lo & care, ...), a key
+// costs two v_bitop3 and one v_bcnt:
+//   mismatches = popcount((lc ^ klo) | (hc ^ khi) | oc).
+// General table (IUPAC codes): records [mask[class][NW] for T, C, A, G, other,
+// bias, key index]: mismatches = popcount(OR_class plane_class & mask_class).
+// bias = -(threshold + 1): popcount + bias < 0 is a hit.
 template <int NW, bool GEN>
-struct Win {
-    uint32_t p[GEN ? 5 : 3][NW];
-    __device__ __forceinline__ Win(const uint32_t (&l)[NW], const uint32_t (&h)[NW], const uint32_t (&o)[NW]) {
-#pragma unroll
-        for (int w = 0; w < NW; w++) {
-            if (GEN) {
-                p[kClsT][w] = ~(l[w] | h[w] | o[w]);
-                p[kClsC][w] = l[w] & ~h[w];
-                p[kClsA][w] = h[w] & ~l[w];
-                p[kClsG][w] = l[w] & h[w];
-                p[kClsO][w] = o[w];
-            } else {
-                p[0][w] = l[w];
-                p[1][w] = h[w];
-                p[2][w] = o[w];
-            }
-        }
-    }
+struct KeyFmt {
+    static constexpr int stride = key_stride(NW, GEN);
+    static constexpr int bias = key_bias_index(NW, GEN);
 };
 
-// popcount(mismatches) + bias: negative iff mismatches <= threshold
-template <int NW, bool GEN>
-__device__ __forceinline__ int32_t key_score(kptr_t kw, const Win<NW, GEN> &win) {
-    uint32_t acc = kw[KeyFmt<NW, GEN>::bias];
+template <int NW>
+__device__ __forceinline__ int32_t score_concrete(kptr_t kw, const uint32_t (&lc)[NW], const uint32_t (&hc)[NW],
+                                                  const uint32_t (&oc)[NW]) {
+    uint32_t acc = kw[2 * NW];
 #pragma unroll
     for (int w = 0; w < NW; w++) {
-        uint32_t x;
-        if (GEN) {
-            x = win.p[kClsO][w] & kw[kClsO * NW + w];
-            x = __builtin_amdgcn_bitop3_b32(win.p[kClsG][w], kw[kClsG * NW + w], x, kAndOr);
-            x = __builtin_amdgcn_bitop3_b32(win.p[kClsA][w], kw[kClsA * NW + w], x, kAndOr);
-            x = __builtin_amdgcn_bitop3_b32(win.p[kClsC][w], kw[kClsC * NW + w], x, kAndOr);
-            x = __builtin_amdgcn_bitop3_b32(win.p[kClsT][w], kw[kClsT * NW + w], x, kAndOr);
-        } else {
-            const uint32_t hx = win.p[1][w] ^ kw[NW + w];
-            x = __builtin_amdgcn_bitop3_b32(win.p[0][w], kw[w], hx, kXorOr);        // (lo^klo) | (hi^khi)
-            x = __builtin_amdgcn_bitop3_b32(x, kw[2 * NW + w], win.p[2][w], kOrAnd);  // (x | ot) & care
-        }
+        const uint32_t x = __builtin_amdgcn_bitop3_b32(hc[w], kw[NW + w], oc[w], kXorOr);  // (hc^khi) | oc
+        acc += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc[w], kw[w], x, kXorOr));    // | (lc^klo)
+    }
+    return (int32_t)acc;
+}
+
+template <int NW>
+__device__ __forceinline__ int32_t score_general(kptr_t kw, const uint32_t (&cls)[5][NW]) {
+    uint32_t acc = kw[5 * NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        uint32_t x = cls[kClsO][w] & kw[kClsO * NW + w];
+        x = __builtin_amdgcn_bitop3_b32(cls[kClsG][w], kw[kClsG * NW + w], x, kAndOr);
+        x = __builtin_amdgcn_bitop3_b32(cls[kClsA][w], kw[kClsA * NW + w], x, kAndOr);
+        x = __builtin_amdgcn_bitop3_b32(cls[kClsC][w], kw[kClsC * NW + w], x, kAndOr);
+        x = __builtin_amdgcn_bitop3_b32(cls[kClsT][w], kw[kClsT * NW + w], x, kAndOr);
         acc += (uint32_t)__popc(x);
     }
     return (int32_t)acc;
 }
 
-// Scores the keys [0, nkeys) of table `kp` against one window and emits hits;
-// four keys per branch on the combined sign.
-template <int NW, bool GEN>
-__device__ __forceinline__ void score_keys(kptr_t kp, uint32_t nkeys, const uint32_t (&l)[NW],
-                                           const uint32_t (&h)[NW], const uint32_t (&o)[NW], uint64_t pos,
-                                           uint32_t strand, uint32_t qid, uint32_t key0, DevHit *hits,
-                                           unsigned long long *count, uint64_t cap) {
-    constexpr int S = KeyFmt<NW, GEN>::stride, B = KeyFmt<NW, GEN>::bias;
-    const Win<NW, GEN> win(l, h, o);
+struct HitSink {
+    DevHit *hits;
+    unsigned long long *count;
+    uint64_t cap;
+    uint64_t pos;
+    uint32_t strand, qid;
+};
+
+// Scores n key records at kp with `score` (record stride S, bias at B, key
+// index at B + 1), eight keys per branch on the OR of their signs.
+template <int S, int B, typename Score>
+__device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const HitSink &o) {
     uint32_t k = 0;
-    for (; k + 4 <= nkeys; k += 4, kp += 4 * S) {
-        const int32_t d0 = key_score<NW, GEN>(kp, win);
-        const int32_t d1 = key_score<NW, GEN>(kp + S, win);
-        const int32_t d2 = key_score<NW, GEN>(kp + 2 * S, win);
-        const int32_t d3 = key_score<NW, GEN>(kp + 3 * S, win);
-        if ((d0 | d1 | d2 | d3) < 0) {
-            if (d0 < 0) emit_hit(hits, count, cap, pos, key0 + k, (uint32_t)(d0 - (int32_t)kp[B]), strand, qid);
-            if (d1 < 0) emit_hit(hits, count, cap, pos, key0 + k + 1, (uint32_t)(d1 - (int32_t)kp[S + B]), strand, qid);
-            if (d2 < 0) emit_hit(hits, count, cap, pos, key0 + k + 2, (uint32_t)(d2 - (int32_t)kp[2 * S + B]), strand, qid);
-            if (d3 < 0) emit_hit(hits, count, cap, pos, key0 + k + 3, (uint32_t)(d3 - (int32_t)kp[3 * S + B]), strand, qid);
+    for (; k + 8 <= n; k += 8, kp += 8 * S) {
+        int32_t d[8];
+#pragma unroll
+        for (int j = 0; j < 8; j++) d[j] = score(kp + j * S);
+        if ((d[0] | d[1] | d[2] | d[3] | d[4] | d[5] | d[6] | d[7]) < 0) {
+#pragma unroll
+            for (int j = 0; j < 8; j++)
+                if (d[j] < 0)
+                    emit_hit(o.hits, o.count, o.cap, o.pos, kp[j * S + B + 1], (uint32_t)(d[j] - (int32_t)kp[j * S + B]),
+                             o.strand, o.qid);
         }
     }
-    for (; k < nkeys; k++, kp += S) {
-        const int32_t d = key_score<NW, GEN>(kp, win);
-        if (d < 0) emit_hit(hits, count, cap, pos, key0 + k, (uint32_t)(d - (int32_t)kp[B]), strand, qid);
+    for (; k < n; k++, kp += S) {
+        const int32_t d = score(kp);
+        if (d < 0) emit_hit(o.hits, o.count, o.cap, o.pos, kp[B + 1], (uint32_t)(d - (int32_t)kp[B]), o.strand, o.qid);
+    }
+}
+
+// Scores a whole key table (nunits = care classes for concrete tables, keys
+// for general ones) against one window.
+template <int NW, bool GEN>
+__device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const uint32_t (&l)[NW],
+                                            const uint32_t (&h)[NW], const uint32_t (&o)[NW], const HitSink &sink) {
+    constexpr int S = KeyFmt<NW, GEN>::stride, B = KeyFmt<NW, GEN>::bias;
+    if (GEN) {
+        uint32_t cls[5][NW];
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            cls[kClsT][w] = ~(l[w] | h[w] | o[w]);
+            cls[kClsC][w] = l[w] & ~h[w];
+            cls[kClsA][w] = h[w] & ~l[w];
+            cls[kClsG][w] = l[w] & h[w];
+            cls[kClsO][w] = o[w];
+        }
+        score_run<S, B>(kp, nunits, [&](kptr_t kw) { return score_general<NW>(kw, cls); }, sink);
+    } else {
+        for (uint32_t c = 0; c < nunits; c++) {
+            uint32_t lc[NW], hc[NW], oc[NW];
+#pragma unroll
+            for (int w = 0; w < NW; w++) {
+                const uint32_t care = kp[w];
+                lc[w] = l[w] & care;
+                hc[w] = h[w] & care;
+                oc[w] = o[w] & care;
+            }
+            const uint32_t n = kp[NW];
+            kp += kClassHdr;
+            score_run<S, B>(kp, n, [&](kptr_t kw) { return score_concrete<NW>(kw, lc, hc, oc); }, sink);
+            kp += n * S;
+        }
     }
 }
 
@@ -290,8 +314,11 @@ struct FinderArgs {
     uint64_t cap;
 };
 
+constexpr int kListCap = 4096;  // LDS candidate entries per pass
+
 __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint32_t s_ent[kListCap];
     __shared__ uint32_t s_wsum[4], s_wsum2[4];
     __shared__ unsigned long long s_base;
     const int t = threadIdx.x;
@@ -305,33 +332,47 @@ __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
     const uint32_t keep = range_mask(s0, a.slab_lo, a.slab_hi) & poison_mask(a.g, a.pd.P, t0, t1, s0);
     fwd &= keep;
     rev &= keep;
-    const uint32_t any = fwd | rev;
+    const uint32_t any = fwd | rev, mine = __popc(any);
     uint32_t total, nstrand;
-    const uint32_t off = block_scan(__popc(any), s_wsum, &total);
+    const uint32_t first = block_scan(mine, s_wsum, &total);
     (void)block_scan(__popc(fwd) + __popc(rev), s_wsum2, &nstrand);
     if (t == 0) {
         s_base = total ? atomicAdd(a.count, (unsigned long long)total) : 0ull;
         if (nstrand) atomicAdd(a.count + 1, (unsigned long long)nstrand);
     }
-    __syncthreads();
-    uint64_t o = s_base + off;
-    uint32_t m = any;
-    while (m) {
-        const uint32_t b = __builtin_ctz(m);
-        m &= m - 1;
-        if (o < a.cap) {
-            a.loci[o] = (uint32_t)(s0 + b - a.slab_lo);
-            const uint32_t fb = (fwd >> b) & 1, rb = (rev >> b) & 1;
-            a.flags[o] = (uint8_t)(fb && rb ? 0 : fb ? 1 : 2);
+    // entries (tile offset << 2 | flag) staged in LDS, then written out coalesced
+    for (uint32_t pass = 0; pass < total; pass += kListCap) {
+        const uint32_t cnt = min(total - pass, (uint32_t)kListCap);
+        if (first < pass + cnt && first + mine > pass) {
+            uint32_t idx = first;
+            uint32_t m = any;
+            while (m) {
+                const uint32_t b = __builtin_ctz(m);
+                m &= m - 1;
+                if (idx >= pass && idx < pass + cnt) {
+                    const uint32_t fb = (fwd >> b) & 1, rb = (rev >> b) & 1;
+                    s_ent[idx - pass] = ((t * 32 + b) << 2) | (fb && rb ? 0u : fb ? 1u : 2u);
+                }
+                idx++;
+            }
         }
-        o++;
+        __syncthreads();
+        for (uint32_t i = t; i < cnt; i += 256) {
+            const uint32_t e = s_ent[i];
+            const uint64_t o = s_base + pass + i;
+            if (o < a.cap) {
+                a.loci[o] = (uint32_t)(t0 + (e >> 2) - a.slab_lo);
+                a.flags[o] = (uint8_t)(e & 3);
+            }
+        }
+        __syncthreads();
     }
 }
 
 struct ComparerArgs {
     DevGenome g;
-    uint32_t P, cbase, key_index, qid;
-    const uint32_t *key;  // one key record
+    uint32_t P, cbase, qid, pad;
+    const uint32_t *table;  // key table of one key (one care class of one key for concrete tables)
     uint64_t slab_lo;
     const uint32_t *loci;
     const uint8_t *flags;
@@ -360,7 +401,7 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
                                         : (int64_t)pos + (int64_t)a.P - 32 * NW - (int64_t)a.cbase;
         uint32_t l[NW], h[NW], o[NW];
         window_planes<NW>(rd, bit, strand, l, h, o);
-        score_keys<NW, GEN>((kptr_t)a.key, 1, l, h, o, pos, strand, a.qid, a.key_index, a.hits, a.count, a.cap);
+        score_table<NW, GEN>((kptr_t)a.table, 1, l, h, o, HitSink{a.hits, a.count, a.cap, pos, strand, a.qid});
     }
 }
 
@@ -369,16 +410,14 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 struct FusedArgs {
     DevGenome g;
     PatternDesc pd;
-    uint32_t cbase, nkeys, qid, pad;
-    const uint32_t *keys;  // nkeys key records
-    uint64_t limit;        // valid window starts are [0, limit)
+    uint32_t cbase, nunits, qid, pad;
+    const uint32_t *table;  // key table (nunits care classes, or nunits general records)
+    uint64_t limit;         // valid window starts are [0, limit)
     DevHit *hits;
     unsigned long long *count;
     unsigned long long *cand_count;
     uint64_t cap;
 };
-
-constexpr int kListCap = 4096;  // LDS candidate entries per pass
 
 template <int NW, bool GEN>
 __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
@@ -437,8 +476,8 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
                                                 : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
                 uint32_t l[NW], h[NW], o[NW];
                 window_planes<NW>(rd, bit, strand, l, h, o);
-                score_keys<NW, GEN>((kptr_t)a.keys, a.nkeys, l, h, o, t0 + ls, strand, a.qid, 0, a.hits, a.count,
-                                    a.cap);
+                score_table<NW, GEN>((kptr_t)a.table, a.nunits, l, h, o,
+                                     HitSink{a.hits, a.count, a.cap, t0 + ls, strand, a.qid});
             }
         }
         __syncthreads();
@@ -457,12 +496,11 @@ hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t sla
     return hipGetLastError();
 }
 
-hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *key,
-                           uint32_t key_index, uint32_t qid, uint64_t slab_lo, const uint32_t *loci,
-                           const uint8_t *flags, uint64_t n, DevHit *hits, unsigned long long *count, uint64_t cap,
-                           hipStream_t s) {
+hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
+                           uint32_t qid, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
+                           DevHit *hits, unsigned long long *count, uint64_t cap, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    ComparerArgs a{g, P, cbase, key_index, qid, key, slab_lo, loci, flags, n, hits, count, cap};
+    ComparerArgs a{g, P, cbase, qid, 0, table, slab_lo, loci, flags, n, hits, count, cap};
     const dim3 grid((uint32_t)((n + 255) / 256)), block(256);
     if (nw == 1) {
         if (gen) hipLaunchKernelGGL((comparer_kernel<1, true>), grid, block, 0, s, a);
@@ -475,17 +513,17 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 }
 
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
-                        const uint32_t *keys, uint32_t nkeys, uint32_t qid, DevHit *hits, unsigned long long *count,
+                        const uint32_t *table, uint32_t nunits, uint32_t qid, DevHit *hits, unsigned long long *count,
                         unsigned long long *cand_count, uint64_t cap, hipStream_t s) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
-    if (limit == 0) return hipSuccess;
+    if (limit == 0 || nunits == 0) return hipSuccess;
     FusedArgs a{};
     a.g = g;
     a.pd = pd;
     a.cbase = cbase;
-    a.nkeys = nkeys;
+    a.nunits = nunits;
     a.qid = qid;
-    a.keys = keys;
+    a.table = table;
     a.limit = limit;
     a.hits = hits;
     a.count = count;

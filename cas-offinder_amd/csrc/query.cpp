@@ -249,11 +249,13 @@ int encode_query(Query &q, std::string &err) {
             const char c = k.seq[i];
             if (mismatch_classes(c) && c != 'A' && c != 'C' && c != 'G' && c != 'T') q.gen = true;
         }
-    const int nw = q.nw, stride = key_stride(nw, q.gen);
-    q.key_words.assign(q.keys.size() * (size_t)stride, 0);
+    const int nw = q.nw, stride = key_stride(nw, q.gen), bi = key_bias_index(nw, q.gen);
+    // one record per key, and its care mask
+    std::vector<std::vector<uint32_t>> rec(q.keys.size(), std::vector<uint32_t>(stride, 0));
+    std::vector<std::vector<uint32_t>> care(q.keys.size(), std::vector<uint32_t>(nw, 0));
     q.care_sum = 0;
     for (size_t n = 0; n < q.keys.size(); n++) {
-        uint32_t *w = &q.key_words[n * (size_t)stride];
+        uint32_t *w = rec[n].data();
         const std::string &s = q.keys[n].seq;
         for (uint32_t i = 0; i < P; i++) {
             const uint32_t mm = mismatch_classes(s[i]);
@@ -261,18 +263,53 @@ int encode_query(Query &q, std::string &err) {
             q.care_sum++;
             const int b = (int)i - q.cbase;
             const uint32_t bit = 1u << (b & 31);
+            care[n][b >> 5] |= bit;
             if (q.gen) {
                 for (int c = 0; c < 5; c++)
                     if (mm & (1u << c)) w[c * nw + (b >> 5)] |= bit;
             } else {
                 // 2-bit code of the key base: T=0, C=1, A=2, G=3
                 const uint32_t code = s[i] == 'T' ? 0 : s[i] == 'C' ? 1 : s[i] == 'A' ? 2 : 3;
-                if (code & 1) w[(b >> 5)] |= bit;
+                if (code & 1) w[b >> 5] |= bit;
                 if (code & 2) w[nw + (b >> 5)] |= bit;
-                w[2 * nw + (b >> 5)] |= bit;
             }
         }
-        w[key_bias_index(nw, q.gen)] = (uint32_t)(-(int64_t)q.keys[n].threshold - 1);
+        w[bi] = (uint32_t)(-(int64_t)q.keys[n].threshold - 1);
+        w[bi + 1] = (uint32_t)n;
+    }
+    q.key_words.clear();
+    q.single_words.clear();
+    if (q.gen) {
+        for (auto &r : rec) q.key_words.insert(q.key_words.end(), r.begin(), r.end());
+        q.key_units = (uint32_t)q.keys.size();
+        q.single_words = q.key_words;
+        q.single_stride = stride;
+    } else {
+        // care classes in order of first appearance
+        std::map<std::vector<uint32_t>, std::vector<size_t>> cls;
+        std::vector<std::vector<uint32_t>> order;
+        for (size_t n = 0; n < q.keys.size(); n++) {
+            auto it = cls.find(care[n]);
+            if (it == cls.end()) { order.push_back(care[n]); cls[care[n]].push_back(n); }
+            else it->second.push_back(n);
+        }
+        for (auto &cm : order) {
+            const auto &members = cls[cm];
+            uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
+            for (int w = 0; w < nw; w++) hdr[w] = cm[w];
+            hdr[nw] = (uint32_t)members.size();
+            q.key_words.insert(q.key_words.end(), hdr, hdr + kClassHdr);
+            for (size_t n : members) q.key_words.insert(q.key_words.end(), rec[n].begin(), rec[n].end());
+        }
+        q.key_units = (uint32_t)order.size();
+        q.single_stride = kClassHdr + stride;
+        for (size_t n = 0; n < q.keys.size(); n++) {
+            uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
+            for (int w = 0; w < nw; w++) hdr[w] = care[n][w];
+            hdr[nw] = 1;
+            q.single_words.insert(q.single_words.end(), hdr, hdr + kClassHdr);
+            q.single_words.insert(q.single_words.end(), rec[n].begin(), rec[n].end());
+        }
     }
     return 0;
 }

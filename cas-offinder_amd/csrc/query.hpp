@@ -34,7 +34,10 @@ struct Query {
     int nw = 1;
     bool gen = false;  // some key has an IUPAC code at a comparable position
     int cbase = 0;
-    std::vector<uint32_t> key_words;  // keys.size() * key_stride(nw, gen)
+    std::vector<uint32_t> key_words;     // fused-kernel table: key_units care classes / records
+    uint32_t key_units = 0;
+    std::vector<uint32_t> single_words;  // one single-key table per key (plain comparer)
+    int single_stride = 0;               // dwords per single-key table
     uint64_t guides = 0;              // guide lines read
     uint64_t care_sum = 0;            // sum over keys of positions that can mismatch
 };

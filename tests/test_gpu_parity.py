@@ -120,7 +120,7 @@ def test_semicolon_and_odd_bytes_in_fasta(ctx, tmp_path):
     gp = str(tmp_path / "odd.fa")
     with open(gp, "w") as f:
         f.write(">x y z\nACGTACGTTTGACCAGGTAC;GGTACCGATTAGGCCAAGG\nAC-GTAC*GTNNNNRYACGTAGGCCAAGGAAGGTT\r\n\n>e\n>w\nacgtAGGtt\n")
-    text = "%s\nNNNNNNNRG 1 1\nACGTACGNN 4\nCCAAGGANN 3 second\n" % gp
+    text = "%s\nNNNNNNNRG 1 1\nACGTACGNN 4 first\nCCAAGGANN 3 second\n" % gp
     check(ctx, tmp_path, gp, text, "odd")
 
 
