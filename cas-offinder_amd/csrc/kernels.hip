@@ -207,16 +207,29 @@ struct KeyFmt {
     static constexpr int bias = key_bias_index(NW, GEN);
 };
 
+struct alignas(16) Rec4 {
+    uint32_t x, y, z, w;
+};
+typedef const __attribute__((address_space(4))) Rec4 *kptr4_t;
+
+// Concrete record read as aligned 16-byte rows (so eight records are two
+// s_load_dwordx16): NW = 1 is {klo, khi, bias, index}; NW = 2 is
+// {klo0, klo1, khi0, khi1}, {bias, index, 0, 0}.
 template <int NW>
 __device__ __forceinline__ int32_t score_concrete(kptr_t kw, const uint32_t (&lc)[NW], const uint32_t (&hc)[NW],
                                                   const uint32_t (&oc)[NW]) {
-    uint32_t acc = kw[2 * NW];
-#pragma unroll
-    for (int w = 0; w < NW; w++) {
-        const uint32_t x = __builtin_amdgcn_bitop3_b32(hc[w], kw[NW + w], oc[w], kXorOr);  // (hc^khi) | oc
-        acc += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc[w], kw[w], x, kXorOr));    // | (lc^klo)
+    const kptr4_t r = (kptr4_t)kw;
+    if (NW == 1) {
+        const uint32_t klo = r[0].x, khi = r[0].y, bias = r[0].z;
+        const uint32_t x = __builtin_amdgcn_bitop3_b32(hc[0], khi, oc[0], kXorOr);  // (hc^khi) | oc
+        return (int32_t)(bias + (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc[0], klo, x, kXorOr)));
+    } else {
+        const uint32_t klo0 = r[0].x, klo1 = r[0].y, khi0 = r[0].z, khi1 = r[0].w, bias = r[1].x;
+        const uint32_t x0 = __builtin_amdgcn_bitop3_b32(hc[0], khi0, oc[0], kXorOr);
+        const uint32_t x1 = __builtin_amdgcn_bitop3_b32(hc[NW - 1], khi1, oc[NW - 1], kXorOr);
+        const uint32_t acc = bias + (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc[0], klo0, x0, kXorOr));
+        return (int32_t)(acc + (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc[NW - 1], klo1, x1, kXorOr)));
     }
-    return (int32_t)acc;
 }
 
 template <int NW>
