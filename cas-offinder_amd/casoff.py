@@ -48,6 +48,7 @@ class Stats(C.Structure):
         ("genome_len", C.c_uint64),
         ("n_base_cmp", C.c_uint64),
         ("n_model_ops", C.c_uint64),
+        ("t_collect_ms", C.c_double),
     ]
 
     def as_dict(self):

@@ -40,7 +40,12 @@ constexpr int kClassHdr = 4;
 constexpr int key_stride(int nw, bool gen) { return gen ? (nw == 1 ? 8 : 12) : (nw == 1 ? 4 : 8); }
 constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 
-// Device hit record, identical to cof_hit of casoff.h.
+// Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.
+constexpr int kHitKeyShift = 40;
+constexpr uint64_t kMaxGenomePositions = 1ull << (kHitKeyShift - 1);
+constexpr uint32_t kMaxSearchKeys = 1u << (64 - kHitKeyShift);
+
+// Host-side hit record, identical to cof_hit of casoff.h.
 struct DevHit {
     uint64_t pos;
     uint32_t key;

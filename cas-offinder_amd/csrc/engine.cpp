@@ -38,7 +38,7 @@ void key_tables(const std::vector<std::shared_ptr<Query>> &qs, std::vector<uint3
 }
 
 int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
-                 uint64_t &nhits, std::string &err) {
+                 const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
     double t_find = 0, t_cmp = 0;
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0, launches = 0;
     std::vector<uint32_t> kw;
@@ -78,8 +78,8 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
             HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
             for (size_t k = 0; k < q.keys.size(); k++) {
                 HIP_TRY(launch_comparer(dg, q.P, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi] + k * (size_t)stride,
-                                        (uint32_t)qi, lo, L.d_loci.p, L.d_flags.p, found[0], L.d_hits.p, L.d_count.p,
-                                        L.d_hits.n, L.stream),
+                                        gbase[qi], lo, L.d_loci.p, L.d_flags.p, found[0], L.d_hkey.p, L.d_hmm.p,
+                                        L.d_count.p, L.d_hkey.n, L.stream),
                         "comparer launch");
                 launches++;
             }
@@ -103,8 +103,8 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     return COF_OK;
 }
 
-int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs, uint64_t &nhits,
-                   std::string &err) {
+int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
+                   const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
     std::vector<uint32_t> kw;
     std::vector<size_t> off;
     key_tables(qs, kw, off, false);
@@ -116,8 +116,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     for (size_t qi = 0; qi < qs.size(); qi++) {
         const Query &q = *qs[qi];
         if (q.keys.empty()) continue;
-        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], q.key_units, (uint32_t)qi,
-                             L.d_hits.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hits.n, L.stream),
+        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], q.key_units, gbase[qi],
+                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hkey.n, L.stream),
                 "fused launch");
         launches++;
     }
@@ -158,7 +158,13 @@ hipError_t Lane::init() {
 
 void Lane::release() {
     if (stream) (void)hipStreamSynchronize(stream);
-    d_hits.release();
+    d_hkey.release();
+    d_hkey2.release();
+    d_hmm.release();
+    d_hmm2.release();
+    d_sort_tmp.release();
+    h_hkey.release();
+    h_hmm.release();
     d_count.release();
     d_loci.release();
     d_flags.release();
@@ -174,36 +180,78 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     *out = nullptr;
     if (qs.size() > 255) { err = "At most 255 queries per search."; return COF_E_INVALID; }
     if (mode != COF_MODE_PLAIN && mode != COF_MODE_BATCHED) { err = "Unknown search mode."; return COF_E_INVALID; }
+    if (dg.total > kMaxGenomePositions) { err = "Genome too large for the hit key layout."; return COF_E_INVALID; }
+    // global compare-key index base of every query
+    std::vector<uint32_t> gbase(qs.size() + 1, 0);
+    uint64_t nkeys = 0;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        gbase[qi] = (uint32_t)nkeys;
+        nkeys += qs[qi]->keys.size();
+        if (nkeys >= kMaxSearchKeys) { err = "Too many compare keys in one search."; return COF_E_INVALID; }
+    }
+    gbase[qs.size()] = (uint32_t)nkeys;
     const auto t0 = std::chrono::steady_clock::now();
     HIP_TRY(L.d_count.reserve(3 + qs.size()), "hipMalloc(counters)");
-    if (!L.d_hits.p) HIP_TRY(L.d_hits.reserve(1 << 20), "hipMalloc(hits)");
+    if (!L.d_hkey.p) {
+        HIP_TRY(L.d_hkey.reserve(1 << 20), "hipMalloc(hits)");
+        HIP_TRY(L.d_hmm.reserve(1 << 20), "hipMalloc(hits)");
+    }
     uint64_t nhits = 0;
     for (int attempt = 0; attempt < 4; attempt++) {
         HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
-        int rc = mode == COF_MODE_PLAIN ? search_plain(L, dg, slab_positions, qs, nhits, err)
-                                        : search_batched(L, dg, qs, nhits, err);
+        int rc = mode == COF_MODE_PLAIN ? search_plain(L, dg, slab_positions, qs, gbase, nhits, err)
+                                        : search_batched(L, dg, qs, gbase, nhits, err);
         if (rc) return rc;
-        if (nhits <= L.d_hits.n) break;
-        HIP_TRY(L.d_hits.reserve(nhits + nhits / 4 + 1024), "hipMalloc(hits)");  // overflow: grow and rerun
+        if (nhits <= L.d_hkey.n) break;
+        const uint64_t want = nhits + nhits / 4 + 1024;  // overflow: grow and rerun
+        HIP_TRY(L.d_hkey.reserve(want), "hipMalloc(hits)");
+        HIP_TRY(L.d_hmm.reserve(want), "hipMalloc(hits)");
     }
-    if (nhits > L.d_hits.n) { err = "Hit buffer overflow."; return COF_E_NOMEM; }
+    if (nhits > L.d_hkey.n) { err = "Hit buffer overflow."; return COF_E_NOMEM; }
+    const auto t_kern = std::chrono::steady_clock::now();
+    // sort on the device by (global key, start, strand), copy to pinned memory
+    const uint64_t *hk = L.d_hkey.p;
+    const uint16_t *hm = L.d_hmm.p;
+    if (nhits > 1) {
+        unsigned keybits = 1;
+        while ((1ull << keybits) < (uint64_t)nkeys) keybits++;
+        const unsigned end_bit = kHitKeyShift + keybits;
+        size_t tmp_bytes = 0;
+        HIP_TRY(sort_hits(nullptr, &tmp_bytes, L.d_hkey.p, L.d_hkey2.p, L.d_hmm.p, L.d_hmm2.p, nhits, end_bit, L.stream),
+                "radix sort size");
+        HIP_TRY(L.d_sort_tmp.reserve(tmp_bytes), "hipMalloc(sort)");
+        HIP_TRY(L.d_hkey2.reserve(nhits), "hipMalloc(sort)");
+        HIP_TRY(L.d_hmm2.reserve(nhits), "hipMalloc(sort)");
+        HIP_TRY(sort_hits(L.d_sort_tmp.p, &tmp_bytes, L.d_hkey.p, L.d_hkey2.p, L.d_hmm.p, L.d_hmm2.p, nhits, end_bit,
+                          L.stream),
+                "radix sort");
+        hk = L.d_hkey2.p;
+        hm = L.d_hmm2.p;
+    }
+    HIP_TRY(L.h_hkey.reserve(nhits), "hipHostMalloc(hits)");
+    HIP_TRY(L.h_hmm.reserve(nhits), "hipHostMalloc(hits)");
+    if (nhits) {
+        HIP_TRY(hipMemcpyAsync(L.h_hkey.p, hk, nhits * 8, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
+        HIP_TRY(hipMemcpyAsync(L.h_hmm.p, hm, nhits * 2, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
+    }
+    HIP_TRY(hipStreamSynchronize(L.stream), "hit copy");
     auto *r = new cof_result();
     r->hits.resize(nhits);
-    static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
-    if (nhits) {
-        hipError_t e = hipMemcpyAsync(r->hits.data(), L.d_hits.p, nhits * sizeof(cof_hit), hipMemcpyDeviceToHost, L.stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(L.stream);
-        if (e != hipSuccess) {
-            delete r;
-            err = std::string("hipMemcpy(hits): ") + hipGetErrorString(e);
-            return COF_E_DEVICE;
+    const uint64_t *hkp = L.h_hkey.p;
+    const uint16_t *hmp = L.h_hmm.p;
+    parallel_for(nhits, 1 << 16, [&](uint64_t b, uint64_t e) {
+        size_t q = 0;
+        for (uint64_t i = b; i < e; i++) {
+            const uint64_t k = hkp[i];
+            const uint32_t gk = (uint32_t)(k >> kHitKeyShift);
+            while (q + 1 < qs.size() && gk >= gbase[q + 1]) q++;  // keys are sorted
+            cof_hit &h = r->hits[i];
+            h.pos = (k >> 1) & (kMaxGenomePositions - 1);
+            h.strand = (uint8_t)(k & 1);
+            h.key = gk - gbase[q];
+            h.mm = hmp[i];
+            h.query = (uint8_t)q;
         }
-    }
-    std::sort(r->hits.begin(), r->hits.end(), [](const cof_hit &a, const cof_hit &b) {
-        if (a.query != b.query) return a.query < b.query;
-        if (a.key != b.key) return a.key < b.key;
-        if (a.pos != b.pos) return a.pos < b.pos;
-        return a.strand < b.strand;
     });
     r->queries = qs;
     r->qbegin.assign(qs.size() + 1, 0);
@@ -215,6 +263,7 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     r->qbegin[qs.size()] = r->hits.size();
     const auto t1 = std::chrono::steady_clock::now();
     L.stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
+    L.stats.t_collect_ms = std::chrono::duration<double, std::milli>(t1 - t_kern).count();
     L.stats.n_hits = nhits;
     L.stats.genome_len = dg.total;
     *out = r;

@@ -29,11 +29,32 @@ namespace cof {
 hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
                          uint32_t *loci, uint8_t *flags, unsigned long long *count, uint64_t cap, hipStream_t s);
 hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
-                           uint32_t qid, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
-                           DevHit *hits, unsigned long long *count, uint64_t cap, hipStream_t s);
+                           uint32_t gbase, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
+                           uint64_t *hkey, uint16_t *hmm, unsigned long long *count, uint64_t cap, hipStream_t s);
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
-                        const uint32_t *table, uint32_t nunits, uint32_t qid, DevHit *hits, unsigned long long *count,
-                        unsigned long long *cand_count, uint64_t cap, hipStream_t s);
+                        const uint32_t *table, uint32_t nunits, uint32_t gbase, uint64_t *hkey, uint16_t *hmm,
+                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s);
+hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
+                     uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
+
+// Pinned host buffer (hipHostMalloc), grow-only.
+template <typename T>
+struct HostBuf {
+    T *p = nullptr;
+    uint64_t n = 0;
+    hipError_t reserve(uint64_t want) {
+        if (want <= n && p) return hipSuccess;
+        if (p) { (void)hipHostFree(p); p = nullptr; n = 0; }
+        hipError_t e = hipHostMalloc((void **)&p, std::max<uint64_t>(want, 1) * sizeof(T), hipHostMallocDefault);
+        if (e == hipSuccess) n = std::max<uint64_t>(want, 1);
+        return e;
+    }
+    void release() {
+        if (p) (void)hipHostFree(p);
+        p = nullptr;
+        n = 0;
+    }
+};
 
 template <typename T>
 struct DevBuf {
@@ -57,7 +78,11 @@ struct DevBuf {
 // may run concurrently over the same resident genome.
 struct Lane {
     hipStream_t stream = nullptr;
-    DevBuf<DevHit> d_hits;
+    DevBuf<uint64_t> d_hkey, d_hkey2;  // hit sort keys (emitted, sorted)
+    DevBuf<uint16_t> d_hmm, d_hmm2;    // hit mismatch counts (emitted, sorted)
+    DevBuf<uint8_t> d_sort_tmp;
+    HostBuf<uint64_t> h_hkey;
+    HostBuf<uint16_t> h_hmm;
     DevBuf<unsigned long long> d_count;  // [0] hits, [1..2] plain finder counters, [2+q] batched candidates
     DevBuf<uint32_t> d_loci;
     DevBuf<uint8_t> d_flags;

@@ -26,6 +26,8 @@
 // complemented key against the forward text (cas-offinder.cl:123-154).
 #include <hip/hip_runtime.h>
 
+#include <rocprim/device/device_radix_sort.hpp>
+
 #include "devtypes.hpp"
 
 namespace cof {
@@ -143,17 +145,22 @@ __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uin
     return off + x - v;
 }
 
-__device__ __forceinline__ void emit_hit(DevHit *hits, unsigned long long *count, uint64_t cap, uint64_t pos,
-                                         uint32_t key, uint32_t mm, uint32_t strand, uint32_t qid) {
-    const unsigned long long slot = atomicAdd(count, 1ull);
-    if (slot < cap) {
-        DevHit h;
-        h.pos = pos;
-        h.key = key;
-        h.mm = (uint16_t)mm;
-        h.strand = (uint8_t)strand;
-        h.query = (uint8_t)qid;
-        hits[slot] = h;
+// Where a workgroup appends hits: a 64-bit sort key
+// (global key index << 40 | window start << 1 | strand) and the mismatch count.
+struct HitSink {
+    uint64_t *hkey;
+    uint16_t *hmm;
+    unsigned long long *count;
+    uint64_t cap;
+    uint64_t pos;
+    uint32_t strand, gbase;
+};
+
+__device__ __forceinline__ void emit_hit(const HitSink &o, uint32_t key, uint32_t mm) {
+    const unsigned long long slot = atomicAdd(o.count, 1ull);
+    if (slot < o.cap) {
+        o.hkey[slot] = ((uint64_t)(o.gbase + key) << kHitKeyShift) | (o.pos << 1) | o.strand;
+        o.hmm[slot] = (uint16_t)mm;
     }
 }
 
@@ -247,35 +254,33 @@ __device__ __forceinline__ int32_t score_general(kptr_t kw, const uint32_t (&cls
     return (int32_t)acc;
 }
 
-struct HitSink {
-    DevHit *hits;
-    unsigned long long *count;
-    uint64_t cap;
-    uint64_t pos;
-    uint32_t strand, qid;
-};
-
 // Scores n key records at kp with `score` (record stride S, bias at B, key
-// index at B + 1), eight keys per branch on the OR of their signs.
+// index at B + 1).  Scores of 32 keys are OR-ed into one sign test; only a
+// block in which some lane has a hit is scored again key by key to emit.
+template <int S, int B, typename Score>
+__device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &score, const HitSink &o) {
+    for (uint32_t k = 0; k < n; k++, kp += S) {
+        const int32_t d = score(kp);
+        if (d < 0) emit_hit(o, kp[B + 1], (uint32_t)(d - (int32_t)kp[B]));
+    }
+}
+
 template <int S, int B, typename Score>
 __device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const HitSink &o) {
     uint32_t k = 0;
+    for (; k + 32 <= n; k += 32, kp += 32 * S) {
+        int32_t acc = 0;
+#pragma unroll
+        for (int j = 0; j < 32; j++) acc |= score(kp + j * S);
+        if (acc < 0) emit_block<S, B>(kp, 32, score, o);
+    }
     for (; k + 8 <= n; k += 8, kp += 8 * S) {
-        int32_t d[8];
+        int32_t acc = 0;
 #pragma unroll
-        for (int j = 0; j < 8; j++) d[j] = score(kp + j * S);
-        if ((d[0] | d[1] | d[2] | d[3] | d[4] | d[5] | d[6] | d[7]) < 0) {
-#pragma unroll
-            for (int j = 0; j < 8; j++)
-                if (d[j] < 0)
-                    emit_hit(o.hits, o.count, o.cap, o.pos, kp[j * S + B + 1], (uint32_t)(d[j] - (int32_t)kp[j * S + B]),
-                             o.strand, o.qid);
-        }
+        for (int j = 0; j < 8; j++) acc |= score(kp + j * S);
+        if (acc < 0) emit_block<S, B>(kp, 8, score, o);
     }
-    for (; k < n; k++, kp += S) {
-        const int32_t d = score(kp);
-        if (d < 0) emit_hit(o.hits, o.count, o.cap, o.pos, kp[B + 1], (uint32_t)(d - (int32_t)kp[B]), o.strand, o.qid);
-    }
+    if (k < n) emit_block<S, B>(kp, n - k, score, o);
 }
 
 // Scores a whole key table (nunits = care classes for concrete tables, keys
@@ -384,13 +389,14 @@ __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
 
 struct ComparerArgs {
     DevGenome g;
-    uint32_t P, cbase, qid, pad;
+    uint32_t P, cbase, gbase, pad;
     const uint32_t *table;  // key table of one key (one care class of one key for concrete tables)
     uint64_t slab_lo;
     const uint32_t *loci;
     const uint8_t *flags;
     uint64_t n;
-    DevHit *hits;
+    uint64_t *hkey;
+    uint16_t *hmm;
     unsigned long long *count;
     uint64_t cap;
 };
@@ -414,7 +420,7 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
                                         : (int64_t)pos + (int64_t)a.P - 32 * NW - (int64_t)a.cbase;
         uint32_t l[NW], h[NW], o[NW];
         window_planes<NW>(rd, bit, strand, l, h, o);
-        score_table<NW, GEN>((kptr_t)a.table, 1, l, h, o, HitSink{a.hits, a.count, a.cap, pos, strand, a.qid});
+        score_table<NW, GEN>((kptr_t)a.table, 1, l, h, o, HitSink{a.hkey, a.hmm, a.count, a.cap, pos, strand, a.gbase});
     }
 }
 
@@ -423,10 +429,11 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 struct FusedArgs {
     DevGenome g;
     PatternDesc pd;
-    uint32_t cbase, nunits, qid, pad;
+    uint32_t cbase, nunits, gbase, pad;
     const uint32_t *table;  // key table (nunits care classes, or nunits general records)
     uint64_t limit;         // valid window starts are [0, limit)
-    DevHit *hits;
+    uint64_t *hkey;
+    uint16_t *hmm;
     unsigned long long *count;
     unsigned long long *cand_count;
     uint64_t cap;
@@ -460,23 +467,21 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
         h = s_hi[wi];
         o = s_ot[wi];
     };
+    // candidates as one 64-bit mask: bit b < 32 forward start b, bit 32 + b reverse start b
+    const uint64_t both = (uint64_t)fwd | ((uint64_t)rev << 32);
     for (uint32_t pass = 0; pass < n; pass += kListCap) {
         const uint32_t cnt = min(n - pass, (uint32_t)kListCap);
-        if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
+        if (n <= kListCap) {  // the usual case: every entry fits, no range checks
             uint32_t idx = first;
-            uint32_t m = fwd;
-            while (m) {
-                const uint32_t b = __builtin_ctz(m);
-                m &= m - 1;
-                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)((t * 32 + b) << 1);
-                idx++;
+            for (uint64_t m = both; m; m &= m - 1, idx++) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                s_list[idx] = (uint16_t)(((t * 32 + (b & 31)) << 1) | (b >> 5));
             }
-            m = rev;
-            while (m) {
-                const uint32_t b = __builtin_ctz(m);
-                m &= m - 1;
-                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)(((t * 32 + b) << 1) | 1);
-                idx++;
+        } else if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
+            uint32_t idx = first;
+            for (uint64_t m = both; m; m &= m - 1, idx++) {
+                const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)(((t * 32 + (b & 31)) << 1) | (b >> 5));
             }
         }
         __syncthreads();
@@ -490,7 +495,7 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
                 uint32_t l[NW], h[NW], o[NW];
                 window_planes<NW>(rd, bit, strand, l, h, o);
                 score_table<NW, GEN>((kptr_t)a.table, a.nunits, l, h, o,
-                                     HitSink{a.hits, a.count, a.cap, t0 + ls, strand, a.qid});
+                                     HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase});
             }
         }
         __syncthreads();
@@ -510,10 +515,10 @@ hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t sla
 }
 
 hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
-                           uint32_t qid, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
-                           DevHit *hits, unsigned long long *count, uint64_t cap, hipStream_t s) {
+                           uint32_t gbase, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
+                           uint64_t *hkey, uint16_t *hmm, unsigned long long *count, uint64_t cap, hipStream_t s) {
     if (n == 0) return hipSuccess;
-    ComparerArgs a{g, P, cbase, qid, 0, table, slab_lo, loci, flags, n, hits, count, cap};
+    ComparerArgs a{g, P, cbase, gbase, 0, table, slab_lo, loci, flags, n, hkey, hmm, count, cap};
     const dim3 grid((uint32_t)((n + 255) / 256)), block(256);
     if (nw == 1) {
         if (gen) hipLaunchKernelGGL((comparer_kernel<1, true>), grid, block, 0, s, a);
@@ -526,8 +531,8 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 }
 
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
-                        const uint32_t *table, uint32_t nunits, uint32_t qid, DevHit *hits, unsigned long long *count,
-                        unsigned long long *cand_count, uint64_t cap, hipStream_t s) {
+                        const uint32_t *table, uint32_t nunits, uint32_t gbase, uint64_t *hkey, uint16_t *hmm,
+                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0 || nunits == 0) return hipSuccess;
     FusedArgs a{};
@@ -535,10 +540,11 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     a.pd = pd;
     a.cbase = cbase;
     a.nunits = nunits;
-    a.qid = qid;
+    a.gbase = gbase;
     a.table = table;
     a.limit = limit;
-    a.hits = hits;
+    a.hkey = hkey;
+    a.hmm = hmm;
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
@@ -551,6 +557,13 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
         else hipLaunchKernelGGL((fused_kernel<2, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+// Radix sort of the hit records by their 64-bit key (bits [0, end_bit)).
+// With tmp == nullptr only *tmp_bytes is set.
+hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
+                     uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s) {
+    return rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u, end_bit, s);
 }
 
 }  // namespace cof

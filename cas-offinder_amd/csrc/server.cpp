@@ -35,9 +35,10 @@ struct Request {
 };
 
 // Admission estimate: the query's key table plus a hit allowance of 4096
-// records per compare key.
+// records per compare key, at 20 device bytes per hit (8-byte sort key and
+// 2-byte mismatch count, emitted and sorted copies).
 uint64_t estimate_bytes(const Query &q) {
-    return q.key_words.size() * 4ull + q.keys.size() * 4096ull * sizeof(DevHit);
+    return q.key_words.size() * 4ull + q.keys.size() * 4096ull * 20ull;
 }
 
 }  // namespace

@@ -62,6 +62,7 @@ typedef struct {
     uint64_t n_base_cmp;    /* sum over pairs of the key's comparable (non-N) positions */
     uint64_t n_model_ops;   /* int32 VALU ops of the bit-sliced compare: 3 per (pair, 32-position
                                word) for A/C/G/T keys, 6 for keys with IUPAC codes */
+    double t_collect_ms;    /* device sort of the hits + copy to host + decode */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference
