@@ -9,7 +9,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libcasoff.so")
+LIB_PATH = os.environ.get("COF_LIB") or os.path.join(HERE, "libcasoff.so")
 CLI_PATH = os.path.join(HERE, "cas-offinder")
 
 MODE_PLAIN = 0

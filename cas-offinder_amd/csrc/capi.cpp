@@ -54,6 +54,7 @@ int cof_close(cof_ctx *c) {
     (void)hipSetDevice(c->device);
     c->lane.release();
     c->d_lo.release(); c->d_hi.release(); c->d_ot.release(); c->d_poison.release();
+    c->d_tile_ppos.release(); c->d_tile_pidx.release();
     delete c;
     return COF_OK;
 }

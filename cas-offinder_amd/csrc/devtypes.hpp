@@ -10,7 +10,9 @@ namespace cof {
 enum : uint32_t { kClsT = 0, kClsC = 1, kClsA = 2, kClsG = 3, kClsO = 4 };
 
 constexpr int kMaxPattern = 64;   // longest padded pattern the kernels take
-constexpr int kTileWords = 256;   // one 32-position word per thread of a 256-thread workgroup
+constexpr int kThreads = 256;     // workgroup size
+constexpr int kWordsPerThread = 4;
+constexpr int kTileWords = kThreads * kWordsPerThread;  // 32-position words per workgroup tile
 constexpr int kTilePos = kTileWords * 32;
 constexpr int kHaloL = 2;         // words staged before the tile
 constexpr int kHaloR = 6;         // words staged after the tile
@@ -57,6 +59,8 @@ struct DevHit {
 struct DevGenome {
     const uint32_t *lo, *hi, *ot;  // planes, nwords + padding words
     const uint64_t *poison;        // sorted; poison[npoison] is a sentinel >= total + kMaxPattern
+    const uint64_t *tile_ppos;     // per kTilePos tile: first poison position >= the tile start
+    const uint32_t *tile_pidx;     // ... and its index in `poison`
     uint64_t npoison;
     uint64_t total;
     uint64_t nwords;

@@ -284,6 +284,21 @@ int cof_ctx::upload_genome() {
     pz.push_back(~0ull);  // sentinel
     HIP_TRY(d_poison.reserve(pz.size()), "hipMalloc(poison)");
     HIP_TRY(hipMemcpy(d_poison.p, pz.data(), pz.size() * 8, hipMemcpyHostToDevice), "hipMemcpy(poison)");
+    // per tile: first poison at or after the tile start, and its index
+    const uint64_t ntiles = g.total / kTilePos + 2;
+    std::vector<uint64_t> tpos(ntiles);
+    std::vector<uint32_t> tidx(ntiles);
+    for (uint64_t t = 0; t < ntiles; t++) {
+        const size_t i = std::lower_bound(g.poison.begin(), g.poison.end(), t * (uint64_t)kTilePos) - g.poison.begin();
+        tidx[t] = (uint32_t)i;
+        tpos[t] = pz[i];
+    }
+    HIP_TRY(d_tile_ppos.reserve(ntiles), "hipMalloc(tiles)");
+    HIP_TRY(d_tile_pidx.reserve(ntiles), "hipMalloc(tiles)");
+    HIP_TRY(hipMemcpy(d_tile_ppos.p, tpos.data(), ntiles * 8, hipMemcpyHostToDevice), "hipMemcpy(tiles)");
+    HIP_TRY(hipMemcpy(d_tile_pidx.p, tidx.data(), ntiles * 4, hipMemcpyHostToDevice), "hipMemcpy(tiles)");
+    dg.tile_ppos = d_tile_ppos.p;
+    dg.tile_pidx = d_tile_pidx.p;
     dg.lo = d_lo.p;
     dg.hi = d_hi.p;
     dg.ot = d_ot.p;

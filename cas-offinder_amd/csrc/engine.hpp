@@ -105,7 +105,8 @@ struct cof_ctx {
     std::string err;
     cof::HostGenome genome;
     cof::DevBuf<uint32_t> d_lo, d_hi, d_ot;
-    cof::DevBuf<uint64_t> d_poison;
+    cof::DevBuf<uint64_t> d_poison, d_tile_ppos;
+    cof::DevBuf<uint32_t> d_tile_pidx;
     cof::DevGenome dg{};
     bool genome_ready = false;
     cof::Lane lane;                        // the context's own lane

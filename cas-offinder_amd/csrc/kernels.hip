@@ -88,28 +88,27 @@ __device__ __forceinline__ uint32_t range_mask(uint64_t s0, uint64_t lo, uint64_
     return m;
 }
 
-// Clears the starts of word [s0, s0+32) whose window [s, s+P) holds a poison
-// position.  t0 / t1: the start range the workgroup covers.
-__device__ __forceinline__ uint32_t poison_mask(const DevGenome &g, uint32_t P, uint64_t t0, uint64_t t1,
-                                                uint64_t s0) {
-    uint64_t lo = 0, n = g.npoison;  // first poison >= t0 (uniform binary search)
-    while (n > 0) {
-        const uint64_t half = n >> 1;
-        if (g.poison[lo + half] < t0) { lo += half + 1; n -= half + 1; }
-        else n = half;
-    }
-    uint32_t keep = ~0u;
-    for (uint64_t i = lo;; i++) {
+// Clears, in the masks of the words starting at s0[j], the starts whose window
+// [s, s+P) holds a poison position.  The per-tile table gives the first poison
+// at or after the tile start t0, so a tile without one costs one scalar load.
+template <int NJ>
+__device__ __forceinline__ void apply_poison(const DevGenome &g, uint32_t P, uint64_t tile, uint64_t t1,
+                                             const uint64_t (&s0)[NJ], uint32_t (&keep)[NJ]) {
+    const uint64_t end = t1 + P - 1;  // poisons at or past `end` touch no start of this tile
+    if (g.tile_ppos[tile] >= end) return;
+    for (uint64_t i = g.tile_pidx[tile];; i++) {
         const uint64_t x = g.poison[i];  // poison[npoison] is the sentinel
-        if (x >= t1 + P - 1) break;
+        if (x >= end) break;
         const uint64_t a = x + 1 >= P ? x + 1 - P : 0;  // starts [a, x] see x
-        if (x >= s0 && a < s0 + 32) {
-            const uint32_t lb = a > s0 ? (uint32_t)(a - s0) : 0u;
-            const uint32_t hb = x < s0 + 31 ? (uint32_t)(x - s0) : 31u;
-            keep &= ~(uint32_t)(((2ull << hb) - 1) & ~((1ull << lb) - 1));
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            if (x >= s0[j] && a < s0[j] + 32) {
+                const uint32_t lb = a > s0[j] ? (uint32_t)(a - s0[j]) : 0u;
+                const uint32_t hb = x < s0[j] + 31 ? (uint32_t)(x - s0[j]) : 31u;
+                keep[j] &= ~(uint32_t)(((2ull << hb) - 1) & ~((1ull << lb) - 1));
+            }
         }
     }
-    return keep;
 }
 
 __device__ __forceinline__ void stage_tile(const DevGenome &g, int64_t w0, uint32_t *s_lo, uint32_t *s_hi,
@@ -123,7 +122,7 @@ __device__ __forceinline__ void stage_tile(const DevGenome &g, int64_t w0, uint3
     }
 }
 
-// exclusive scan of `v` over the 256-thread workgroup; *total gets the sum
+// exclusive scan of `v` over the workgroup (four wave64s); *total gets the sum
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total) {
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t x = v;
@@ -254,65 +253,96 @@ __device__ __forceinline__ int32_t score_general(kptr_t kw, const uint32_t (&cls
     return (int32_t)acc;
 }
 
-// Scores n key records at kp with `score` (record stride S, bias at B, key
-// index at B + 1).  Scores of 32 keys are OR-ed into one sign test; only a
-// block in which some lane has a hit is scored again key by key to emit.
-template <int S, int B, typename Score>
-__device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &score, const HitSink &o) {
+// Scores n key records at kp against W windows at once (each scalar-loaded
+// key is applied to every window: register blocking).  score(kw, w) is the
+// biased score of window w; record stride S, bias at B, key index at B + 1.
+// Scores of 32 keys are OR-ed into one sign test; only a block in which some
+// lane has a hit is scored again key by key to emit.  Window w emits only if
+// valid[w] (an invalid slot duplicates a valid window of the same lane).
+template <int W, int S, int B, typename Score>
+__device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &score, const HitSink (&o)[W],
+                                           const bool (&valid)[W]) {
     for (uint32_t k = 0; k < n; k++, kp += S) {
-        const int32_t d = score(kp);
-        if (d < 0) emit_hit(o, kp[B + 1], (uint32_t)(d - (int32_t)kp[B]));
+#pragma unroll
+        for (int w = 0; w < W; w++) {
+            const int32_t d = score(kp, w);
+            if (d < 0 && valid[w]) emit_hit(o[w], kp[B + 1], (uint32_t)(d - (int32_t)kp[B]));
+        }
     }
 }
 
-template <int S, int B, typename Score>
-__device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const HitSink &o) {
+#ifndef COF_BLOCK_KEYS
+#define COF_BLOCK_KEYS 32
+#endif
+#ifndef COF_FUSED_WINDOWS
+#define COF_FUSED_WINDOWS 1
+#endif
+constexpr int kBlockKeys = COF_BLOCK_KEYS;
+
+template <int W, int S, int B, int BLK, typename Score>
+__device__ __forceinline__ bool block_hit(kptr_t kp, const Score &score) {
+    int32_t acc = 0;
+#pragma unroll
+    for (int j = 0; j < BLK; j++) {
+        int32_t d = score(kp + j * S, 0);
+#pragma unroll
+        for (int w = 1; w < W; w++) d |= score(kp + j * S, w);
+        acc |= d;
+    }
+    return acc < 0;
+}
+
+// Tables of the fused kernel hold a multiple of 8 records per run (padded
+// with records that never hit), so only the plain path's one-key tables reach
+// the per-key tail.
+template <int W, int S, int B, typename Score>
+__device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const HitSink (&o)[W],
+                                          const bool (&valid)[W]) {
     uint32_t k = 0;
-    for (; k + 32 <= n; k += 32, kp += 32 * S) {
-        int32_t acc = 0;
-#pragma unroll
-        for (int j = 0; j < 32; j++) acc |= score(kp + j * S);
-        if (acc < 0) emit_block<S, B>(kp, 32, score, o);
-    }
-    for (; k + 8 <= n; k += 8, kp += 8 * S) {
-        int32_t acc = 0;
-#pragma unroll
-        for (int j = 0; j < 8; j++) acc |= score(kp + j * S);
-        if (acc < 0) emit_block<S, B>(kp, 8, score, o);
-    }
-    if (k < n) emit_block<S, B>(kp, n - k, score, o);
+    for (; k + kBlockKeys <= n; k += kBlockKeys, kp += kBlockKeys * S)
+        if (block_hit<W, S, B, kBlockKeys>(kp, score)) emit_block<W, S, B>(kp, kBlockKeys, score, o, valid);
+    for (; k + 8 <= n; k += 8, kp += 8 * S)
+        if (block_hit<W, S, B, 8>(kp, score)) emit_block<W, S, B>(kp, 8, score, o, valid);
+    if (k < n) emit_block<W, S, B>(kp, n - k, score, o, valid);
 }
 
 // Scores a whole key table (nunits = care classes for concrete tables, keys
-// for general ones) against one window.
-template <int NW, bool GEN>
-__device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const uint32_t (&l)[NW],
-                                            const uint32_t (&h)[NW], const uint32_t (&o)[NW], const HitSink &sink) {
+// for general ones) against W windows (planes l, h, o of each).
+template <int NW, bool GEN, int W>
+__device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const uint32_t (&l)[W][NW],
+                                            const uint32_t (&h)[W][NW], const uint32_t (&o)[W][NW],
+                                            const HitSink (&sink)[W], const bool (&valid)[W]) {
     constexpr int S = KeyFmt<NW, GEN>::stride, B = KeyFmt<NW, GEN>::bias;
     if (GEN) {
-        uint32_t cls[5][NW];
+        uint32_t cls[W][5][NW];
 #pragma unroll
-        for (int w = 0; w < NW; w++) {
-            cls[kClsT][w] = ~(l[w] | h[w] | o[w]);
-            cls[kClsC][w] = l[w] & ~h[w];
-            cls[kClsA][w] = h[w] & ~l[w];
-            cls[kClsG][w] = l[w] & h[w];
-            cls[kClsO][w] = o[w];
-        }
-        score_run<S, B>(kp, nunits, [&](kptr_t kw) { return score_general<NW>(kw, cls); }, sink);
+        for (int w = 0; w < W; w++)
+#pragma unroll
+            for (int x = 0; x < NW; x++) {
+                cls[w][kClsT][x] = ~(l[w][x] | h[w][x] | o[w][x]);
+                cls[w][kClsC][x] = l[w][x] & ~h[w][x];
+                cls[w][kClsA][x] = h[w][x] & ~l[w][x];
+                cls[w][kClsG][x] = l[w][x] & h[w][x];
+                cls[w][kClsO][x] = o[w][x];
+            }
+        score_run<W, S, B>(kp, nunits, [&](kptr_t kw, int w) { return score_general<NW>(kw, cls[w]); }, sink, valid);
     } else {
         for (uint32_t c = 0; c < nunits; c++) {
-            uint32_t lc[NW], hc[NW], oc[NW];
+            uint32_t lc[W][NW], hc[W][NW], oc[W][NW];
 #pragma unroll
-            for (int w = 0; w < NW; w++) {
-                const uint32_t care = kp[w];
-                lc[w] = l[w] & care;
-                hc[w] = h[w] & care;
-                oc[w] = o[w] & care;
+            for (int x = 0; x < NW; x++) {
+                const uint32_t care = kp[x];
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    lc[w][x] = l[w][x] & care;
+                    hc[w][x] = h[w][x] & care;
+                    oc[w][x] = o[w][x] & care;
+                }
             }
             const uint32_t n = kp[NW];
             kp += kClassHdr;
-            score_run<S, B>(kp, n, [&](kptr_t kw) { return score_concrete<NW>(kw, lc, hc, oc); }, sink);
+            score_run<W, S, B>(kp, n, [&](kptr_t kw, int w) { return score_concrete<NW>(kw, lc[w], hc[w], oc[w]); },
+                               sink, valid);
             kp += n * S;
         }
     }
@@ -334,7 +364,7 @@ struct FinderArgs {
 
 constexpr int kListCap = 4096;  // LDS candidate entries per pass
 
-__global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
+__global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint32_t s_ent[kListCap];
     __shared__ uint32_t s_wsum[4], s_wsum2[4];
@@ -344,16 +374,27 @@ __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
     stage_tile(a.g, w0, s_lo, s_hi, s_ot);
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
-    const uint64_t s0 = t0 + 32ull * t;
-    uint32_t fwd, rev;
-    finder_masks(a.pd, s_lo, s_hi, s_ot, t + kHaloL, &fwd, &rev);
-    const uint32_t keep = range_mask(s0, a.slab_lo, a.slab_hi) & poison_mask(a.g, a.pd.P, t0, t1, s0);
-    fwd &= keep;
-    rev &= keep;
-    const uint32_t any = fwd | rev, mine = __popc(any);
+    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
+    uint64_t s0[kWordsPerThread];
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {  // thread t owns tile words j * kThreads + t
+        const int wi = j * kThreads + t;
+        s0[j] = t0 + 32ull * wi;
+        finder_masks(a.pd, s_lo, s_hi, s_ot, wi + kHaloL, &fwd[j], &rev[j]);
+        keep[j] = range_mask(s0[j], a.slab_lo, a.slab_hi);
+    }
+    apply_poison(a.g, a.pd.P, t0 / kTilePos, t1, s0, keep);
+    uint32_t mine = 0, nmine = 0;
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {
+        fwd[j] &= keep[j];
+        rev[j] &= keep[j];
+        mine += __popc(fwd[j] | rev[j]);
+        nmine += __popc(fwd[j]) + __popc(rev[j]);
+    }
     uint32_t total, nstrand;
     const uint32_t first = block_scan(mine, s_wsum, &total);
-    (void)block_scan(__popc(fwd) + __popc(rev), s_wsum2, &nstrand);
+    (void)block_scan(nmine, s_wsum2, &nstrand);
     if (t == 0) {
         s_base = total ? atomicAdd(a.count, (unsigned long long)total) : 0ull;
         if (nstrand) atomicAdd(a.count + 1, (unsigned long long)nstrand);
@@ -363,19 +404,20 @@ __global__ __launch_bounds__(256) void finder_kernel(FinderArgs a) {
         const uint32_t cnt = min(total - pass, (uint32_t)kListCap);
         if (first < pass + cnt && first + mine > pass) {
             uint32_t idx = first;
-            uint32_t m = any;
-            while (m) {
-                const uint32_t b = __builtin_ctz(m);
-                m &= m - 1;
-                if (idx >= pass && idx < pass + cnt) {
-                    const uint32_t fb = (fwd >> b) & 1, rb = (rev >> b) & 1;
-                    s_ent[idx - pass] = ((t * 32 + b) << 2) | (fb && rb ? 0u : fb ? 1u : 2u);
+#pragma unroll
+            for (int j = 0; j < kWordsPerThread; j++) {
+                const uint32_t wi = (uint32_t)(j * kThreads + t);
+                for (uint32_t m = fwd[j] | rev[j]; m; m &= m - 1, idx++) {
+                    const uint32_t b = __builtin_ctz(m);
+                    if (idx >= pass && idx < pass + cnt) {
+                        const uint32_t fb = (fwd[j] >> b) & 1, rb = (rev[j] >> b) & 1;
+                        s_ent[idx - pass] = ((wi * 32 + b) << 2) | (fb && rb ? 0u : fb ? 1u : 2u);
+                    }
                 }
-                idx++;
             }
         }
         __syncthreads();
-        for (uint32_t i = t; i < cnt; i += 256) {
+        for (uint32_t i = t; i < cnt; i += kThreads) {
             const uint32_t e = s_ent[i];
             const uint64_t o = s_base + pass + i;
             if (o < a.cap) {
@@ -418,9 +460,11 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
         if (strand == 0 ? !(fl == 0 || fl == 1) : !(fl == 0 || fl == 2)) continue;
         const int64_t bit = strand == 0 ? (int64_t)pos + a.cbase
                                         : (int64_t)pos + (int64_t)a.P - 32 * NW - (int64_t)a.cbase;
-        uint32_t l[NW], h[NW], o[NW];
-        window_planes<NW>(rd, bit, strand, l, h, o);
-        score_table<NW, GEN>((kptr_t)a.table, 1, l, h, o, HitSink{a.hkey, a.hmm, a.count, a.cap, pos, strand, a.gbase});
+        uint32_t l[1][NW], h[1][NW], o[1][NW];
+        window_planes<NW>(rd, bit, strand, l[0], h[0], o[0]);
+        const HitSink sink[1] = {HitSink{a.hkey, a.hmm, a.count, a.cap, pos, strand, a.gbase}};
+        const bool valid[1] = {true};
+        score_table<NW, GEN, 1>((kptr_t)a.table, 1, l, h, o, sink, valid);
     }
 }
 
@@ -440,7 +484,7 @@ struct FusedArgs {
 };
 
 template <int NW, bool GEN>
-__global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
+__global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
@@ -449,15 +493,25 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
     stage_tile(a.g, w0, s_lo, s_hi, s_ot);
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
-    const uint64_t s0 = t0 + 32ull * t;
     const uint32_t P = a.pd.P;
-    uint32_t fwd, rev;
-    finder_masks(a.pd, s_lo, s_hi, s_ot, t + kHaloL, &fwd, &rev);
-    const uint32_t keep = range_mask(s0, 0, a.limit) & poison_mask(a.g, P, t0, t1, s0);
-    fwd &= keep;
-    rev &= keep;
+    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
+    uint64_t s0[kWordsPerThread];
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {  // thread t owns tile words j * kThreads + t
+        const int wi = j * kThreads + t;
+        s0[j] = t0 + 32ull * wi;
+        finder_masks(a.pd, s_lo, s_hi, s_ot, wi + kHaloL, &fwd[j], &rev[j]);
+        keep[j] = range_mask(s0[j], 0, a.limit);
+    }
+    apply_poison(a.g, P, blockIdx.x, t1, s0, keep);
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {
+        fwd[j] &= keep[j];
+        rev[j] &= keep[j];
+        mine += __popc(fwd[j]) + __popc(rev[j]);
+    }
     uint32_t n;
-    const uint32_t mine = __popc(fwd) + __popc(rev);
     const uint32_t first = block_scan(mine, s_wsum, &n);
     if (t == 0 && n) atomicAdd(a.cand_count, (unsigned long long)n);
 
@@ -467,35 +521,44 @@ __global__ __launch_bounds__(256) void fused_kernel(FusedArgs a) {
         h = s_hi[wi];
         o = s_ot[wi];
     };
-    // candidates as one 64-bit mask: bit b < 32 forward start b, bit 32 + b reverse start b
-    const uint64_t both = (uint64_t)fwd | ((uint64_t)rev << 32);
+    // list entries: tile offset << 1 | strand (forward entries of a word first)
     for (uint32_t pass = 0; pass < n; pass += kListCap) {
         const uint32_t cnt = min(n - pass, (uint32_t)kListCap);
-        if (n <= kListCap) {  // the usual case: every entry fits, no range checks
+        if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
+            const bool all = n <= kListCap;
             uint32_t idx = first;
-            for (uint64_t m = both; m; m &= m - 1, idx++) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                s_list[idx] = (uint16_t)(((t * 32 + (b & 31)) << 1) | (b >> 5));
-            }
-        } else if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
-            uint32_t idx = first;
-            for (uint64_t m = both; m; m &= m - 1, idx++) {
-                const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                if (idx >= pass && idx < pass + cnt) s_list[idx - pass] = (uint16_t)(((t * 32 + (b & 31)) << 1) | (b >> 5));
+#pragma unroll
+            for (int j = 0; j < kWordsPerThread; j++) {
+                const uint32_t wi = (uint32_t)(j * kThreads + t);
+                for (uint64_t m = (uint64_t)fwd[j] | ((uint64_t)rev[j] << 32); m; m &= m - 1, idx++) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                    if (all || (idx >= pass && idx < pass + cnt))
+                        s_list[idx - pass] = (uint16_t)(((wi * 32 + (b & 31)) << 1) | (b >> 5));
+                }
             }
         }
         __syncthreads();
-        for (uint32_t base = (uint32_t)wave * 64; base < cnt; base += 256) {
-            const uint32_t i = base + lane;
-            if (i < cnt) {
-                const uint32_t e = s_list[i];
-                const uint32_t ls = e >> 1, strand = e & 1;
-                const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
-                                                : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
-                uint32_t l[NW], h[NW], o[NW];
-                window_planes<NW>(rd, bit, strand, l, h, o);
-                score_table<NW, GEN>((kptr_t)a.table, a.nunits, l, h, o,
-                                     HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase});
+        // each lane scores W candidates (register blocking); chunks of 64*W
+        // candidates go round-robin over the four waves
+        constexpr int W = COF_FUSED_WINDOWS;
+        for (uint32_t base = (uint32_t)wave * 64 * W; base < cnt; base += kThreads * W) {
+            const uint32_t i0 = base + lane;
+            if (i0 < cnt) {
+                uint32_t l[W][NW], h[W][NW], o[W][NW];
+                HitSink sink[W];
+                bool valid[W];
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    const uint32_t iw = i0 + 64 * w;
+                    valid[w] = iw < cnt;
+                    const uint32_t e = s_list[valid[w] ? iw : i0];  // an empty slot repeats candidate i0
+                    const uint32_t ls = e >> 1, strand = e & 1;
+                    const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
+                                                    : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
+                    window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
+                    sink[w] = HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase};
+                }
+                score_table<NW, GEN, W>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
             }
         }
         __syncthreads();
@@ -510,7 +573,7 @@ hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t sla
     if (slab_hi <= slab_lo) return hipSuccess;
     FinderArgs a{g, pd, slab_lo, slab_hi, loci, flags, count, cap};
     const uint64_t tiles = (slab_hi - slab_lo + kTilePos - 1) / kTilePos;
-    hipLaunchKernelGGL(finder_kernel, dim3((uint32_t)tiles), dim3(256), 0, s, a);
+    hipLaunchKernelGGL(finder_kernel, dim3((uint32_t)tiles), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 
@@ -548,7 +611,7 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
-    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos)), block(256);
+    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos)), block(kThreads);
     if (nw == 1) {
         if (gen) hipLaunchKernelGGL((fused_kernel<1, true>), grid, block, 0, s, a);
         else hipLaunchKernelGGL((fused_kernel<1, false>), grid, block, 0, s, a);

@@ -279,9 +279,16 @@ int encode_query(Query &q, std::string &err) {
     }
     q.key_words.clear();
     q.single_words.clear();
+    // never-hit padding record: bias 2^30 keeps every score non-negative
+    std::vector<uint32_t> pad(stride, 0);
+    pad[bi] = 1u << 30;
+    auto pad8 = [&](size_t n) {
+        for (size_t i = n; i % 8; i++) q.key_words.insert(q.key_words.end(), pad.begin(), pad.end());
+    };
     if (q.gen) {
         for (auto &r : rec) q.key_words.insert(q.key_words.end(), r.begin(), r.end());
-        q.key_units = (uint32_t)q.keys.size();
+        pad8(rec.size());
+        q.key_units = (uint32_t)((rec.size() + 7) / 8 * 8);
         q.single_words = q.key_words;
         q.single_stride = stride;
     } else {
@@ -297,9 +304,10 @@ int encode_query(Query &q, std::string &err) {
             const auto &members = cls[cm];
             uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
             for (int w = 0; w < nw; w++) hdr[w] = cm[w];
-            hdr[nw] = (uint32_t)members.size();
+            hdr[nw] = (uint32_t)((members.size() + 7) / 8 * 8);
             q.key_words.insert(q.key_words.end(), hdr, hdr + kClassHdr);
             for (size_t n : members) q.key_words.insert(q.key_words.end(), rec[n].begin(), rec[n].end());
+            pad8(members.size());
         }
         q.key_units = (uint32_t)order.size();
         q.single_stride = kClassHdr + stride;
