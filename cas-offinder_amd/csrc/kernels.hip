@@ -51,33 +51,44 @@ __device__ __forceinline__ uint32_t class_match(uint32_t m, uint32_t l, uint32_t
            (G & sel_mask(m, kClsG)) | (o & sel_mask(m, kClsO));
 }
 
-// PAM test of the 32 starts of LDS word i0 (words i0+1, i0+2 give the pattern
-// reach): bit b of *fwd / *rev = start b passes the forward / reverse
-// complement pattern.
+// PAM test of the kWordsPerThread consecutive LDS words starting at i0 (the
+// two words after them give the pattern reach): bit b of fwd[j] / rev[j] =
+// start b of word i0 + j passes the forward / reverse-complement pattern.
 __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32_t *s_lo, const uint32_t *s_hi,
-                                             const uint32_t *s_ot, int i0, uint32_t *fwd, uint32_t *rev) {
-    const uint32_t L0 = s_lo[i0], L1 = s_lo[i0 + 1], L2 = s_lo[i0 + 2];
-    const uint32_t H0 = s_hi[i0], H1 = s_hi[i0 + 1], H2 = s_hi[i0 + 2];
-    const uint32_t O0 = s_ot[i0], O1 = s_ot[i0 + 1], O2 = s_ot[i0 + 2];
-    uint32_t f = ~0u, r = ~0u;
+                                             const uint32_t *s_ot, int i0, uint32_t (&fwd)[kWordsPerThread],
+                                             uint32_t (&rev)[kWordsPerThread]) {
+    constexpr int NJ = kWordsPerThread, NL = kWordsPerThread + 2;
+    uint32_t L[NL], H[NL], O[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) {
+        L[i] = s_lo[i0 + i];
+        H[i] = s_hi[i0 + i];
+        O[i] = s_ot[i0 + i];
+    }
+#pragma unroll
+    for (int j = 0; j < NJ; j++) fwd[j] = rev[j] = ~0u;
     for (uint32_t i = 0; i < pd.nf; i++) {
-        const uint32_t k = pd.fk[i], sh = k & 31;
+        const uint32_t k = pd.fk[i], sh = k & 31, m = pd.fm[i];
         const bool up = k >= 32;
-        const uint32_t l = funnel(up ? L2 : L1, up ? L1 : L0, sh);
-        const uint32_t h = funnel(up ? H2 : H1, up ? H1 : H0, sh);
-        const uint32_t o = funnel(up ? O2 : O1, up ? O1 : O0, sh);
-        f &= class_match(pd.fm[i], l, h, o);
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
+            const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
+            const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
+            fwd[j] &= class_match(m, l, h, o);
+        }
     }
     for (uint32_t i = 0; i < pd.nr; i++) {
-        const uint32_t k = pd.rk[i], sh = k & 31;
+        const uint32_t k = pd.rk[i], sh = k & 31, m = pd.rm[i];
         const bool up = k >= 32;
-        const uint32_t l = funnel(up ? L2 : L1, up ? L1 : L0, sh);
-        const uint32_t h = funnel(up ? H2 : H1, up ? H1 : H0, sh);
-        const uint32_t o = funnel(up ? O2 : O1, up ? O1 : O0, sh);
-        r &= class_match(pd.rm[i], l, h, o);
+#pragma unroll
+        for (int j = 0; j < NJ; j++) {
+            const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
+            const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
+            const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
+            rev[j] &= class_match(m, l, h, o);
+        }
     }
-    *fwd = f;
-    *rev = r;
 }
 
 // Bits of the starts [s0, s0+32) inside [lo, hi).
@@ -376,11 +387,10 @@ __global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
     uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
     uint64_t s0[kWordsPerThread];
+    finder_masks(a.pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
 #pragma unroll
-    for (int j = 0; j < kWordsPerThread; j++) {  // thread t owns tile words j * kThreads + t
-        const int wi = j * kThreads + t;
-        s0[j] = t0 + 32ull * wi;
-        finder_masks(a.pd, s_lo, s_hi, s_ot, wi + kHaloL, &fwd[j], &rev[j]);
+    for (int j = 0; j < kWordsPerThread; j++) {
+        s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
         keep[j] = range_mask(s0[j], a.slab_lo, a.slab_hi);
     }
     apply_poison(a.g, a.pd.P, t0 / kTilePos, t1, s0, keep);
@@ -406,7 +416,7 @@ __global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
             uint32_t idx = first;
 #pragma unroll
             for (int j = 0; j < kWordsPerThread; j++) {
-                const uint32_t wi = (uint32_t)(j * kThreads + t);
+                const uint32_t wi = (uint32_t)(kWordsPerThread * t + j);
                 for (uint32_t m = fwd[j] | rev[j]; m; m &= m - 1, idx++) {
                     const uint32_t b = __builtin_ctz(m);
                     if (idx >= pass && idx < pass + cnt) {
@@ -496,11 +506,10 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     const uint32_t P = a.pd.P;
     uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
     uint64_t s0[kWordsPerThread];
+    finder_masks(a.pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
 #pragma unroll
-    for (int j = 0; j < kWordsPerThread; j++) {  // thread t owns tile words j * kThreads + t
-        const int wi = j * kThreads + t;
-        s0[j] = t0 + 32ull * wi;
-        finder_masks(a.pd, s_lo, s_hi, s_ot, wi + kHaloL, &fwd[j], &rev[j]);
+    for (int j = 0; j < kWordsPerThread; j++) {
+        s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
         keep[j] = range_mask(s0[j], 0, a.limit);
     }
     apply_poison(a.g, P, blockIdx.x, t1, s0, keep);
@@ -529,7 +538,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
             uint32_t idx = first;
 #pragma unroll
             for (int j = 0; j < kWordsPerThread; j++) {
-                const uint32_t wi = (uint32_t)(j * kThreads + t);
+                const uint32_t wi = (uint32_t)(kWordsPerThread * t + j);
                 for (uint64_t m = (uint64_t)fwd[j] | ((uint64_t)rev[j] << 32); m; m &= m - 1, idx++) {
                     const uint32_t b = (uint32_t)__builtin_ctzll(m);
                     if (all || (idx >= pass && idx < pass + cnt))
