@@ -54,6 +54,20 @@ def build_workload(name, ctx, n_guides=None):
     return texts, qs, guides, desc
 
 
+def measured_traffic(workload):
+    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (profiles/), or None."""
+    import glob
+    import re
+    for p in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
+        m = re.search(r"<!-- traffic (\{.*\}) -->", open(p).read())
+        if m:
+            d = json.loads(m.group(1))
+            if d.get("workload") == workload:
+                return d["fetch_bytes"] + d["write_bytes"], os.path.relpath(p, ROOT)
+    return None, None
+
+
 def calibrate_copy(nbytes):
     """A device copy of known size (torch elementwise kernel) for scaling the
     FETCH_SIZE / WRITE_SIZE counters."""
@@ -163,6 +177,7 @@ def main():
     achieved = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
     survey = SURVEY_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
     hbm_achieved = 0.375 * total / launch_s / 1e9
+    traffic, traffic_src = measured_traffic(args.workload)
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -188,7 +203,7 @@ def main():
         "genome_generate_upload_s": round(t_load, 2),
         "roofline": {"bound": "valu", "kernel": "fused_kernel", "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": None, "avg_launch_ms": round(avg_launch_ms, 4),
+                     "traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": round(avg_launch_ms, 4),
                      "work_per_launch": "%d int32 ops = 3 per (candidate, key, 32-position word)" % (
                          st_b["n_model_ops"] / max(l_b, 1)),
                      "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
