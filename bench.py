@@ -127,6 +127,43 @@ def cpu_baseline(ctx, texts, target_s=12.0):
                                                                           res["candidates"])}
 
 
+def serve_bench(ctx, n_requests, qps, seed=3, verify=True):
+    """Poisson request stream (workloads.serving_stream) through the
+    continuous-batching server; latency is submit -> records in host memory."""
+    reqs = workloads.serving_stream(n_requests, qps, "synthetic", seed=seed)
+    queries = [[ctx.query(t) for _, t in groups] for _, groups, _ in reqs]
+    n_guides = sum(n for _, _, n in reqs)
+    srv = casoff.Server(ctx, mem_budget_bytes=8 << 30, max_batch_queries=64)
+    tickets = []
+    t0 = time.perf_counter()
+    for (arrival, _, _), qs in zip(reqs, queries):
+        wait = arrival - (time.perf_counter() - t0)
+        if wait > 0:
+            time.sleep(wait)
+        tickets.append(srv.submit(qs))
+    done = [srv.poll(t, wait=True) for t in tickets]
+    wall = time.perf_counter() - t0
+    srv.stop()
+    lat = sorted(ms for _, ms in done)
+    pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
+    out = {"requests": n_requests, "qps_offered": qps, "guides": n_guides,
+           "p50_ms": round(pct(0.50), 2), "p99_ms": round(pct(0.99), 2), "max_ms": round(lat[-1], 2),
+           "wall_s": round(wall, 3), "guide_x_gbp_per_s": round(n_guides * ctx.info()[0] / 1e9 / wall, 2)}
+    if verify:
+        same_alone, same_plain, checked_plain = 0, 0, 0
+        for (res, _), qs, (_, _, n) in zip(done, queries, reqs):
+            got = res.records()
+            alone = ctx.search(qs, casoff.MODE_BATCHED).records()
+            same_alone += int(len(got) == len(alone) and bool((got == alone).all()))
+            if n <= 8:
+                plain = ctx.search(qs, casoff.MODE_PLAIN).records()
+                checked_plain += 1
+                same_plain += int(len(got) == len(plain) and bool((got == plain).all()))
+        out["bit_exact_vs_alone"] = "%d/%d" % (same_alone, n_requests)
+        out["bit_exact_vs_plain"] = "%d/%d (requests of <= 8 guides)" % (same_plain, checked_plain)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +175,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--genome-len", type=int, default=3_100_000_000)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
+    ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0: skip)")
+    ap.add_argument("--serve-qps", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -211,6 +250,10 @@ def main():
                      "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
     }
+    if args.serve_requests > 0:
+        log("serving: %d requests at %.1f QPS" % (args.serve_requests, args.serve_qps))
+        out["serving"] = serve_bench(ctx, args.serve_requests, args.serve_qps)
+        log("serving: p50 %.1f ms, p99 %.1f ms" % (out["serving"]["p50_ms"], out["serving"]["p99_ms"]))
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, texts)
     print(json.dumps(out), flush=True)

@@ -4,6 +4,7 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <map>
 
 using namespace cof;
 
@@ -103,26 +104,48 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     return COF_OK;
 }
 
+// Batched path: queries with the same padded pattern form one group that
+// shares one fused launch (one genome pass): their keys go into one table
+// whose records carry global key indices.
 int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
                    const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
+    std::vector<std::vector<size_t>> groups;
+    {
+        std::map<std::string, size_t> gi;
+        for (size_t qi = 0; qi < qs.size(); qi++) {
+            if (qs[qi]->keys.empty()) continue;
+            auto it = gi.find(qs[qi]->pattern);
+            if (it == gi.end()) {
+                gi[qs[qi]->pattern] = groups.size();
+                groups.push_back({qi});
+            } else {
+                groups[it->second].push_back(qi);
+            }
+        }
+    }
+    std::vector<KeyTable> tab(groups.size());
     std::vector<uint32_t> kw;
-    std::vector<size_t> off;
-    key_tables(qs, kw, off, false);
+    std::vector<size_t> off(groups.size());
+    for (size_t g = 0; g < groups.size(); g++) {
+        std::vector<KeyRef> refs;
+        for (size_t qi : groups[g])
+            for (size_t k = 0; k < qs[qi]->keys.size(); k++) refs.push_back({&qs[qi]->keys[k], gbase[qi] + (uint32_t)k});
+        if (build_key_table(refs, qs[groups[g][0]]->P, tab[g], err)) return COF_E_INVALID;
+        off[g] = kw.size();
+        kw.insert(kw.end(), tab[g].words.begin(), tab[g].words.end());
+    }
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-    uint64_t launches = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        const Query &q = *qs[qi];
-        if (q.keys.empty()) continue;
-        HIP_TRY(launch_fused(dg, q.pat, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi], q.key_units, gbase[qi],
-                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + qi, L.d_hkey.n, L.stream),
+    for (size_t g = 0; g < groups.size(); g++) {
+        const Query &rep = *qs[groups[g][0]];
+        HIP_TRY(launch_fused(dg, rep.pat, tab[g].nw, tab[g].gen, (uint32_t)tab[g].cbase, L.d_keys.p + off[g], tab[g].units,
+                             0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + g, L.d_hkey.n, L.stream),
                 "fused launch");
-        launches++;
     }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
-    std::vector<unsigned long long> cnt(2 + qs.size());
+    std::vector<unsigned long long> cnt(2 + groups.size());
     HIP_TRY(hipMemcpyAsync(cnt.data(), L.d_count.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            L.stream),
             "hipMemcpyAsync(count)");
@@ -131,17 +154,18 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        ncand += cnt[2 + qi];
-        npairs += cnt[2 + qi] * qs[qi]->keys.size();
-        nbase += cnt[2 + qi] * qs[qi]->care_sum;
-        nmodel += cnt[2 + qi] * qs[qi]->keys.size() * (uint64_t)qs[qi]->nw * (qs[qi]->gen ? 6 : 3);
+    for (size_t g = 0; g < groups.size(); g++) {
+        const uint64_t c = cnt[2 + g];
+        ncand += c;
+        npairs += c * tab[g].nkeys;
+        nbase += c * tab[g].care_sum;
+        nmodel += c * tab[g].nkeys * (uint64_t)tab[g].nw * (tab[g].gen ? 6 : 3);
     }
     L.stats.n_candidates = ncand;
     L.stats.n_pairs = npairs;
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
-    L.stats.n_launches = launches;
+    L.stats.n_launches = groups.size();
     nhits = cnt[0];
     return COF_OK;
 }

@@ -212,6 +212,90 @@ int parse_query(const char *text, size_t len, Query &q, std::string &err) {
     return 0;
 }
 
+// Records of `keys` at offset cbase / width nw (bit b of word w <-> key
+// position cbase + 32 w + b) in the table format of devtypes.hpp.  With pad8
+// every run (care class, or the general record list) is padded to a multiple
+// of 8 records with never-hit records (bias 2^30).
+void encode_records(const std::vector<KeyRef> &keys, uint32_t P, int cbase, int nw, bool gen, bool pad8,
+                    std::vector<uint32_t> &out, uint32_t &units) {
+    const int stride = key_stride(nw, gen), bi = key_bias_index(nw, gen);
+    std::vector<std::vector<uint32_t>> rec(keys.size(), std::vector<uint32_t>(stride, 0));
+    std::vector<std::vector<uint32_t>> care(keys.size(), std::vector<uint32_t>(nw, 0));
+    for (size_t n = 0; n < keys.size(); n++) {
+        uint32_t *w = rec[n].data();
+        const std::string &s = keys[n].key->seq;
+        for (uint32_t i = 0; i < P; i++) {
+            const uint32_t mm = mismatch_classes(s[i]);
+            if (!mm) continue;
+            const int b = (int)i - cbase;
+            const uint32_t bit = 1u << (b & 31);
+            care[n][b >> 5] |= bit;
+            if (gen) {
+                for (int c = 0; c < 5; c++)
+                    if (mm & (1u << c)) w[c * nw + (b >> 5)] |= bit;
+            } else {
+                // 2-bit code of the key base: T=0, C=1, A=2, G=3
+                const uint32_t code = s[i] == 'T' ? 0 : s[i] == 'C' ? 1 : s[i] == 'A' ? 2 : 3;
+                if (code & 1) w[b >> 5] |= bit;
+                if (code & 2) w[nw + (b >> 5)] |= bit;
+            }
+        }
+        w[bi] = (uint32_t)(-(int64_t)keys[n].key->threshold - 1);
+        w[bi + 1] = keys[n].index;
+    }
+    std::vector<uint32_t> padrec(stride, 0);
+    padrec[bi] = 1u << 30;
+    auto pad = [&](size_t n) {
+        if (pad8)
+            for (size_t i = n; i % 8; i++) out.insert(out.end(), padrec.begin(), padrec.end());
+    };
+    if (gen) {
+        for (auto &r : rec) out.insert(out.end(), r.begin(), r.end());
+        pad(rec.size());
+        units = (uint32_t)(pad8 ? (rec.size() + 7) / 8 * 8 : rec.size());
+        return;
+    }
+    // care classes in order of first appearance
+    std::map<std::vector<uint32_t>, std::vector<size_t>> cls;
+    std::vector<std::vector<uint32_t>> order;
+    for (size_t n = 0; n < keys.size(); n++) {
+        auto it = cls.find(care[n]);
+        if (it == cls.end()) { order.push_back(care[n]); cls[care[n]].push_back(n); }
+        else it->second.push_back(n);
+    }
+    for (auto &cm : order) {
+        const auto &members = cls[cm];
+        uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
+        for (int w = 0; w < nw; w++) hdr[w] = cm[w];
+        hdr[nw] = (uint32_t)(pad8 ? (members.size() + 7) / 8 * 8 : members.size());
+        out.insert(out.end(), hdr, hdr + kClassHdr);
+        for (size_t n : members) out.insert(out.end(), rec[n].begin(), rec[n].end());
+        pad(members.size());
+    }
+    units = (uint32_t)order.size();
+}
+
+int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, std::string &err) {
+    t = KeyTable();
+    int lo = -1, hi = -1;
+    for (auto &k : keys)
+        for (uint32_t i = 0; i < P; i++) {
+            const char c = k.key->seq[i];
+            if (!mismatch_classes(c)) continue;
+            t.care_sum++;
+            if (lo < 0 || (int)i < lo) lo = (int)i;
+            if ((int)i > hi) hi = (int)i;
+            if (c != 'A' && c != 'C' && c != 'G' && c != 'T') t.gen = true;
+        }
+    if (lo < 0) lo = hi = 0;
+    if (hi - lo + 1 > 2 * 32) { err = "Key span too wide."; return -1; }
+    t.cbase = lo;
+    t.nw = hi - lo + 1 <= 32 ? 1 : 2;
+    t.nkeys = keys.size();
+    encode_records(keys, P, t.cbase, t.nw, t.gen, true, t.words, t.units);
+    return 0;
+}
+
 int encode_query(Query &q, std::string &err) {
     const uint32_t P = q.P;
     if (P == 0) { err = "Empty pattern."; return -1; }
@@ -231,93 +315,23 @@ int encode_query(Query &q, std::string &err) {
         mm = mismatch_classes(rc[k]);
         if (mm) { pd.rk[pd.nr] = (uint8_t)k; pd.rm[pd.nr] = (uint8_t)(~mm & 31u); pd.nr++; }
     }
-    // care span of all keys
-    int lo = -1, hi = -1;
-    for (auto &k : q.keys)
-        for (uint32_t i = 0; i < P; i++)
-            if (mismatch_classes(k.seq[i])) {
-                if (lo < 0 || (int)i < lo) lo = (int)i;
-                if ((int)i > hi) hi = (int)i;
-            }
-    if (lo < 0) lo = hi = 0;
-    q.cbase = lo;
-    const int span = hi - lo + 1;
-    q.nw = span <= 32 ? 1 : 2;
-    q.gen = false;
-    for (auto &k : q.keys)
-        for (uint32_t i = 0; i < P; i++) {
-            const char c = k.seq[i];
-            if (mismatch_classes(c) && c != 'A' && c != 'C' && c != 'G' && c != 'T') q.gen = true;
-        }
-    const int nw = q.nw, stride = key_stride(nw, q.gen), bi = key_bias_index(nw, q.gen);
-    // one record per key, and its care mask
-    std::vector<std::vector<uint32_t>> rec(q.keys.size(), std::vector<uint32_t>(stride, 0));
-    std::vector<std::vector<uint32_t>> care(q.keys.size(), std::vector<uint32_t>(nw, 0));
-    q.care_sum = 0;
-    for (size_t n = 0; n < q.keys.size(); n++) {
-        uint32_t *w = rec[n].data();
-        const std::string &s = q.keys[n].seq;
-        for (uint32_t i = 0; i < P; i++) {
-            const uint32_t mm = mismatch_classes(s[i]);
-            if (!mm) continue;
-            q.care_sum++;
-            const int b = (int)i - q.cbase;
-            const uint32_t bit = 1u << (b & 31);
-            care[n][b >> 5] |= bit;
-            if (q.gen) {
-                for (int c = 0; c < 5; c++)
-                    if (mm & (1u << c)) w[c * nw + (b >> 5)] |= bit;
-            } else {
-                // 2-bit code of the key base: T=0, C=1, A=2, G=3
-                const uint32_t code = s[i] == 'T' ? 0 : s[i] == 'C' ? 1 : s[i] == 'A' ? 2 : 3;
-                if (code & 1) w[b >> 5] |= bit;
-                if (code & 2) w[nw + (b >> 5)] |= bit;
-            }
-        }
-        w[bi] = (uint32_t)(-(int64_t)q.keys[n].threshold - 1);
-        w[bi + 1] = (uint32_t)n;
-    }
-    q.key_words.clear();
+    std::vector<KeyRef> refs;
+    for (size_t n = 0; n < q.keys.size(); n++) refs.push_back({&q.keys[n], (uint32_t)n});
+    KeyTable t;
+    if (build_key_table(refs, P, t, err)) return -1;
+    q.key_words.swap(t.words);
+    q.key_units = t.units;
+    q.nw = t.nw;
+    q.gen = t.gen;
+    q.cbase = t.cbase;
+    q.care_sum = t.care_sum;
+    // one single-key table per key for the plain comparer (same offset and format)
+    const int stride = key_stride(q.nw, q.gen);
+    q.single_stride = q.gen ? stride : kClassHdr + stride;
     q.single_words.clear();
-    // never-hit padding record: bias 2^30 keeps every score non-negative
-    std::vector<uint32_t> pad(stride, 0);
-    pad[bi] = 1u << 30;
-    auto pad8 = [&](size_t n) {
-        for (size_t i = n; i % 8; i++) q.key_words.insert(q.key_words.end(), pad.begin(), pad.end());
-    };
-    if (q.gen) {
-        for (auto &r : rec) q.key_words.insert(q.key_words.end(), r.begin(), r.end());
-        pad8(rec.size());
-        q.key_units = (uint32_t)((rec.size() + 7) / 8 * 8);
-        q.single_words = q.key_words;
-        q.single_stride = stride;
-    } else {
-        // care classes in order of first appearance
-        std::map<std::vector<uint32_t>, std::vector<size_t>> cls;
-        std::vector<std::vector<uint32_t>> order;
-        for (size_t n = 0; n < q.keys.size(); n++) {
-            auto it = cls.find(care[n]);
-            if (it == cls.end()) { order.push_back(care[n]); cls[care[n]].push_back(n); }
-            else it->second.push_back(n);
-        }
-        for (auto &cm : order) {
-            const auto &members = cls[cm];
-            uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
-            for (int w = 0; w < nw; w++) hdr[w] = cm[w];
-            hdr[nw] = (uint32_t)((members.size() + 7) / 8 * 8);
-            q.key_words.insert(q.key_words.end(), hdr, hdr + kClassHdr);
-            for (size_t n : members) q.key_words.insert(q.key_words.end(), rec[n].begin(), rec[n].end());
-            pad8(members.size());
-        }
-        q.key_units = (uint32_t)order.size();
-        q.single_stride = kClassHdr + stride;
-        for (size_t n = 0; n < q.keys.size(); n++) {
-            uint32_t hdr[kClassHdr] = {0, 0, 0, 0};
-            for (int w = 0; w < nw; w++) hdr[w] = care[n][w];
-            hdr[nw] = 1;
-            q.single_words.insert(q.single_words.end(), hdr, hdr + kClassHdr);
-            q.single_words.insert(q.single_words.end(), rec[n].begin(), rec[n].end());
-        }
+    for (size_t n = 0; n < q.keys.size(); n++) {  // at the query's offset and width
+        uint32_t units;
+        encode_records({refs[n]}, P, q.cbase, q.nw, q.gen, false, q.single_words, units);
     }
     return 0;
 }

@@ -54,4 +54,21 @@ void revcomp(std::string &s);  // cas-offinder.cpp:35-58 complement table, then 
 int parse_query(const char *text, size_t len, Query &q, std::string &err);
 int encode_query(Query &q, std::string &err);
 
+// Fused-kernel key table of any set of compare keys of one padded pattern
+// length P: common key offset cbase, care classes merged across the keys,
+// every record carrying the caller's key index.
+struct KeyTable {
+    std::vector<uint32_t> words;
+    uint32_t units = 0;  // care classes (concrete) or padded record count (general)
+    int nw = 1;
+    bool gen = false;
+    int cbase = 0;
+    uint64_t nkeys = 0, care_sum = 0;
+};
+struct KeyRef {
+    const CompareKey *key;
+    uint32_t index;  // written into the record; hits report it
+};
+int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, std::string &err);
+
 }  // namespace cof
