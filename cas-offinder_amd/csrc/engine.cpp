@@ -279,12 +279,10 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     });
     r->queries = qs;
     r->qbegin.assign(qs.size() + 1, 0);
-    size_t h = 0;
-    for (size_t qi = 0; qi < qs.size(); qi++) {
-        r->qbegin[qi] = h;
-        while (h < r->hits.size() && r->hits[h].query == qi) h++;
-    }
-    r->qbegin[qs.size()] = r->hits.size();
+    for (size_t qi = 0; qi <= qs.size(); qi++)  // hits are sorted by query
+        r->qbegin[qi] = std::partition_point(r->hits.begin(), r->hits.end(),
+                                             [qi](const cof_hit &x) { return x.query < qi; }) -
+                        r->hits.begin();
     const auto t1 = std::chrono::steady_clock::now();
     L.stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     L.stats.t_collect_ms = std::chrono::duration<double, std::milli>(t1 - t_kern).count();

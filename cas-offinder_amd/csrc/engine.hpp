@@ -7,6 +7,7 @@
 #include <algorithm>
 #include <memory>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/casoff.h"
@@ -18,8 +19,30 @@ struct cof_query {
     std::shared_ptr<cof::Query> q;
 };
 
+namespace cof {
+// Allocator whose value-less construct leaves trivial elements uninitialized, so
+// resizing a hit vector does not zero-fill memory that is written right after.
+template <typename T>
+struct UninitAlloc : std::allocator<T> {
+    template <typename U>
+    struct rebind {
+        using other = UninitAlloc<U>;
+    };
+    UninitAlloc() = default;
+    template <typename U>
+    UninitAlloc(const UninitAlloc<U> &) {}
+    template <typename U, typename... A>
+    void construct(U *p, A &&...a) {
+        if constexpr (sizeof...(A) == 0)
+            ::new ((void *)p) U;
+        else
+            ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+}  // namespace cof
+
 struct cof_result {
-    std::vector<cof_hit> hits;                      // sorted (query, key, pos, strand)
+    std::vector<cof_hit, cof::UninitAlloc<cof_hit>> hits;  // sorted (query, key, pos, strand)
     std::vector<std::shared_ptr<cof::Query>> queries;
     std::vector<size_t> qbegin;                     // hits of query i: [qbegin[i], qbegin[i+1])
 };

@@ -103,12 +103,13 @@ void cof_server::worker(Lane *L) {
                 const size_t nqi = batch[i]->qs.size();
                 p->queries = batch[i]->qs;
                 p->qbegin.assign(nqi + 1, 0);
+                const size_t h0 = res->qbegin[q0];
+                p->hits.resize(res->qbegin[q0 + nqi] - h0);
                 for (size_t j = 0; j < nqi; j++) {
-                    p->qbegin[j] = p->hits.size();
+                    p->qbegin[j] = res->qbegin[q0 + j] - h0;
                     for (size_t h = res->qbegin[q0 + j]; h < res->qbegin[q0 + j + 1]; h++) {
-                        cof_hit x = res->hits[h];
-                        x.query = (uint8_t)j;
-                        p->hits.push_back(x);
+                        p->hits[h - h0] = res->hits[h];
+                        p->hits[h - h0].query = (uint8_t)j;
                     }
                 }
                 p->qbegin[nqi] = p->hits.size();
