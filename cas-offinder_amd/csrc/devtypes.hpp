@@ -42,6 +42,19 @@ constexpr int kClassHdr = 4;
 constexpr int key_stride(int nw, bool gen) { return gen ? (nw == 1 ? 8 : 12) : (nw == 1 ? 4 : 8); }
 constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 
+// Bulge-family tables (device-side DNA/RNA-bulge expansion): a guide is scored
+// at kMaxShifts window alignments at most (its own plus one per bulge family).
+// Records, grouped by care class like concrete key tables ([care[NW], nguides,
+// 0, 0] headers), are [guide planes][variant block offset][bias per shift],
+// padded to 4 dwords; planes are glo[NW], ghi[NW] (concrete) or the five class
+// masks [class * NW + word] (general).  A variant block is [gbase, then per
+// shift: first split (bit coordinate, int32), split count, entry offset] and
+// each entry is [bias, query-local key index].
+constexpr int kMaxShifts = 5;
+constexpr int bulge_planes(int nw, bool gen) { return gen ? 5 * nw : 2 * nw; }
+constexpr int bulge_stride(int nw, bool gen, int ns) { return (bulge_planes(nw, gen) + 1 + ns + 3) & ~3; }
+constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that never hits
+
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.
 constexpr int kHitKeyShift = 40;
 constexpr uint64_t kMaxGenomePositions = 1ull << (kHitKeyShift - 1);

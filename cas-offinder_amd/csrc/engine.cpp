@@ -104,48 +104,91 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     return COF_OK;
 }
 
-// Batched path: queries with the same padded pattern form one group that
-// shares one fused launch (one genome pass): their keys go into one table
-// whose records carry global key indices.
+// Batched path, one fused launch (one genome pass) per group:
+//  - queries with DNA/RNA bulges that share padded pattern, bulge sizes and
+//    PAM side form a bulge group: bulge_kernel expands their variants on the
+//    device from the guides;
+//  - the other queries with the same padded pattern form a key group whose
+//    compare keys go into one table (fused_kernel).
+// Records carry global key indices.
 int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
                    const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
-    std::vector<std::vector<size_t>> groups;
-    {
+    auto group_by = [&](const std::vector<size_t> &idx, auto keyf) {
+        std::vector<std::vector<size_t>> groups;
         std::map<std::string, size_t> gi;
-        for (size_t qi = 0; qi < qs.size(); qi++) {
-            if (qs[qi]->keys.empty()) continue;
-            auto it = gi.find(qs[qi]->pattern);
+        for (size_t qi : idx) {
+            const std::string k = keyf(*qs[qi]);
+            auto it = gi.find(k);
             if (it == gi.end()) {
-                gi[qs[qi]->pattern] = groups.size();
+                gi[k] = groups.size();
                 groups.push_back({qi});
             } else {
                 groups[it->second].push_back(qi);
             }
         }
+        return groups;
+    };
+    std::vector<size_t> bulged, plain;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        if (qs[qi]->keys.empty()) continue;
+        (qs[qi]->bulge.empty() ? plain : bulged).push_back(qi);
     }
-    std::vector<KeyTable> tab(groups.size());
-    std::vector<uint32_t> kw;
-    std::vector<size_t> off(groups.size());
-    for (size_t g = 0; g < groups.size(); g++) {
+    std::vector<BulgeTable> btab;
+    for (auto &grp : group_by(bulged, [](const Query &q) {
+             return q.pattern + "|" + std::to_string(q.D) + "|" + std::to_string(q.R) + (q.rev ? "|r" : "|f");
+         })) {
+        std::vector<BulgeRef> refs;
+        for (size_t qi : grp) refs.push_back({qs[qi].get(), gbase[qi]});
+        BulgeTable t;
+        if (build_bulge_table(refs, t) == 1) btab.push_back(std::move(t));
+        else plain.insert(plain.end(), grp.begin(), grp.end());
+    }
+    std::sort(plain.begin(), plain.end());
+    const auto kgroups = group_by(plain, [](const Query &q) { return q.pattern; });
+    std::vector<KeyTable> ktab(kgroups.size());
+    std::vector<const Query *> krep(kgroups.size()), brep;
+    for (size_t g = 0; g < kgroups.size(); g++) {
         std::vector<KeyRef> refs;
-        for (size_t qi : groups[g])
+        for (size_t qi : kgroups[g])
             for (size_t k = 0; k < qs[qi]->keys.size(); k++) refs.push_back({&qs[qi]->keys[k], gbase[qi] + (uint32_t)k});
-        if (build_key_table(refs, qs[groups[g][0]]->P, tab[g], err)) return COF_E_INVALID;
-        off[g] = kw.size();
-        kw.insert(kw.end(), tab[g].words.begin(), tab[g].words.end());
+        if (build_key_table(refs, qs[kgroups[g][0]]->P, ktab[g], err)) return COF_E_INVALID;
+        krep[g] = qs[kgroups[g][0]].get();
+    }
+    // one device buffer: key tables, then bulge tables
+    std::vector<uint32_t> kw;
+    std::vector<size_t> koff(ktab.size()), boff(btab.size());
+    for (size_t g = 0; g < ktab.size(); g++) {
+        koff[g] = kw.size();
+        kw.insert(kw.end(), ktab[g].words.begin(), ktab[g].words.end());
+        kw.resize((kw.size() + 3) & ~(size_t)3, 0);  // 16-byte aligned tables
+    }
+    for (size_t g = 0; g < btab.size(); g++) {
+        boff[g] = kw.size();
+        kw.insert(kw.end(), btab[g].words.begin(), btab[g].words.end());
+        kw.resize((kw.size() + 3) & ~(size_t)3, 0);
     }
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
+    const size_t nlaunch = ktab.size() + btab.size();
+    // counters: [0] hits, [2 + launch] candidates (run_search sized them for one launch per query)
+    if (2 + nlaunch > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-    for (size_t g = 0; g < groups.size(); g++) {
-        const Query &rep = *qs[groups[g][0]];
-        HIP_TRY(launch_fused(dg, rep.pat, tab[g].nw, tab[g].gen, (uint32_t)tab[g].cbase, L.d_keys.p + off[g], tab[g].units,
-                             0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + g, L.d_hkey.n, L.stream),
+    for (size_t g = 0; g < ktab.size(); g++) {
+        HIP_TRY(launch_fused(dg, krep[g]->pat, ktab[g].nw, ktab[g].gen, (uint32_t)ktab[g].cbase, L.d_keys.p + koff[g],
+                             ktab[g].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + g, L.d_hkey.n,
+                             L.stream),
                 "fused launch");
     }
+    for (size_t g = 0; g < btab.size(); g++) {
+        const BulgeTable &t = btab[g];
+        HIP_TRY(launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[g], t.units,
+                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + ktab.size() + g, L.d_hkey.n,
+                             L.stream),
+                "bulge launch");
+    }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
-    std::vector<unsigned long long> cnt(2 + groups.size());
+    std::vector<unsigned long long> cnt(2 + nlaunch);
     HIP_TRY(hipMemcpyAsync(cnt.data(), L.d_count.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            L.stream),
             "hipMemcpyAsync(count)");
@@ -154,18 +197,29 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
-    for (size_t g = 0; g < groups.size(); g++) {
+    for (size_t g = 0; g < ktab.size(); g++) {
         const uint64_t c = cnt[2 + g];
         ncand += c;
-        npairs += c * tab[g].nkeys;
-        nbase += c * tab[g].care_sum;
-        nmodel += c * tab[g].nkeys * (uint64_t)tab[g].nw * (tab[g].gen ? 6 : 3);
+        npairs += c * ktab[g].nkeys;
+        nbase += c * ktab[g].care_sum;
+        nmodel += c * ktab[g].nkeys * (uint64_t)ktab[g].nw * (ktab[g].gen ? 6 : 3);
+    }
+    for (size_t g = 0; g < btab.size(); g++) {
+        const BulgeTable &t = btab[g];
+        const uint64_t c = cnt[2 + ktab.size() + g];
+        ncand += c;
+        npairs += c * t.nkeys;
+        nbase += c * t.care_sum;
+        // per (candidate, guide): masks at ns shifts, no-bulge popcount, one
+        // AND + popcount per family, OR of the family scores
+        const uint64_t ops = (uint64_t)t.nw * ((t.gen ? 5 : 2) * t.ns + 1 + 2 * (t.ns - 1)) + (t.ns - 1);
+        nmodel += c * t.nguides * ops;
     }
     L.stats.n_candidates = ncand;
     L.stats.n_pairs = npairs;
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
-    L.stats.n_launches = groups.size();
+    L.stats.n_launches = nlaunch;
     nhits = cnt[0];
     return COF_OK;
 }

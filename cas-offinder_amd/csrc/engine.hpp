@@ -57,6 +57,10 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
                         const uint32_t *table, uint32_t nunits, uint32_t gbase, uint64_t *hkey, uint16_t *hmm,
                         unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s);
+hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
+                        const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
+                        uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
+                        hipStream_t s);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
                      uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
 

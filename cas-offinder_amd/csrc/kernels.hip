@@ -480,6 +480,80 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 
 // --------------------------------------------------------------- batched path
 
+// Finder stage of the batched kernels for the tile of this workgroup: PAM
+// masks, poison and range clearing, compaction of (start, strand) entries into
+// an LDS list of kListCap entries per pass, then cands(t0, rd, entry, valid)
+// per lane for chunks of 64 * W candidates (round-robin over the four waves).
+// Entries are tile offset << 1 | strand (forward entries of a word first).
+template <int W, typename Cands>
+__device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
+                                                unsigned long long *cand_count, uint32_t *s_lo, uint32_t *s_hi,
+                                                uint32_t *s_ot, uint16_t *s_list, uint32_t *s_wsum, Cands &&cands) {
+    const int t = threadIdx.x;
+    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
+    stage_tile(g, w0, s_lo, s_hi, s_ot);
+    __syncthreads();
+    const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
+    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
+    uint64_t s0[kWordsPerThread];
+    finder_masks(pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {
+        s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
+        keep[j] = range_mask(s0[j], 0, limit);
+    }
+    apply_poison(g, pd.P, blockIdx.x, t1, s0, keep);
+    uint32_t mine = 0;
+#pragma unroll
+    for (int j = 0; j < kWordsPerThread; j++) {
+        fwd[j] &= keep[j];
+        rev[j] &= keep[j];
+        mine += __popc(fwd[j]) + __popc(rev[j]);
+    }
+    uint32_t n;
+    const uint32_t first = block_scan(mine, s_wsum, &n);
+    if (t == 0 && n) atomicAdd(cand_count, (unsigned long long)n);
+
+    const int lane = t & 63, wave = t >> 6;
+    auto rd = [=](int64_t wi, uint32_t &l, uint32_t &h, uint32_t &o) {
+        l = s_lo[wi];
+        h = s_hi[wi];
+        o = s_ot[wi];
+    };
+    for (uint32_t pass = 0; pass < n; pass += kListCap) {
+        const uint32_t cnt = min(n - pass, (uint32_t)kListCap);
+        if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
+            const bool all = n <= kListCap;
+            uint32_t idx = first;
+#pragma unroll
+            for (int j = 0; j < kWordsPerThread; j++) {
+                const uint32_t wi = (uint32_t)(kWordsPerThread * t + j);
+                for (uint64_t m = (uint64_t)fwd[j] | ((uint64_t)rev[j] << 32); m; m &= m - 1, idx++) {
+                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
+                    if (all || (idx >= pass && idx < pass + cnt))
+                        s_list[idx - pass] = (uint16_t)(((wi * 32 + (b & 31)) << 1) | (b >> 5));
+                }
+            }
+        }
+        __syncthreads();
+        for (uint32_t base = (uint32_t)wave * 64 * W; base < cnt; base += kThreads * W) {
+            const uint32_t i0 = base + lane;
+            if (i0 < cnt) {
+                uint32_t e[W];
+                bool valid[W];
+#pragma unroll
+                for (int w = 0; w < W; w++) {
+                    const uint32_t iw = i0 + 64 * w;
+                    valid[w] = iw < cnt;
+                    e[w] = s_list[valid[w] ? iw : i0];  // an empty slot repeats candidate i0
+                }
+                cands(t0, rd, e, valid);
+            }
+        }
+        __syncthreads();
+    }
+}
+
 struct FusedArgs {
     DevGenome g;
     PatternDesc pd;
@@ -498,80 +572,206 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
-    const int t = threadIdx.x;
-    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
-    stage_tile(a.g, w0, s_lo, s_hi, s_ot);
-    __syncthreads();
-    const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
+    constexpr int W = COF_FUSED_WINDOWS;  // candidates per lane (register blocking)
     const uint32_t P = a.pd.P;
-    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
-    uint64_t s0[kWordsPerThread];
-    finder_masks(a.pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
+    tile_candidates<W>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
+                       [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
+                           uint32_t l[W][NW], h[W][NW], o[W][NW];
+                           HitSink sink[W];
 #pragma unroll
-    for (int j = 0; j < kWordsPerThread; j++) {
-        s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
-        keep[j] = range_mask(s0[j], 0, a.limit);
-    }
-    apply_poison(a.g, P, blockIdx.x, t1, s0, keep);
-    uint32_t mine = 0;
-#pragma unroll
-    for (int j = 0; j < kWordsPerThread; j++) {
-        fwd[j] &= keep[j];
-        rev[j] &= keep[j];
-        mine += __popc(fwd[j]) + __popc(rev[j]);
-    }
-    uint32_t n;
-    const uint32_t first = block_scan(mine, s_wsum, &n);
-    if (t == 0 && n) atomicAdd(a.cand_count, (unsigned long long)n);
+                           for (int w = 0; w < W; w++) {
+                               const uint32_t ls = e[w] >> 1, strand = e[w] & 1;
+                               const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
+                                                               : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase +
+                                                                     32 * kHaloL;
+                               window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
+                               sink[w] = HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase};
+                           }
+                           score_table<NW, GEN, W>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
+                       });
+}
 
-    const int lane = t & 63, wave = t >> 6;
-    auto rd = [&](int64_t wi, uint32_t &l, uint32_t &h, uint32_t &o) {
-        l = s_lo[wi];
-        h = s_hi[wi];
-        o = s_ot[wi];
-    };
-    // list entries: tile offset << 1 | strand (forward entries of a word first)
-    for (uint32_t pass = 0; pass < n; pass += kListCap) {
-        const uint32_t cnt = min(n - pass, (uint32_t)kListCap);
-        if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
-            const bool all = n <= kListCap;
-            uint32_t idx = first;
+// ------------------------------------------------------- device bulge expansion
+//
+// bulge_kernel scores guides instead of compare keys: for each candidate it
+// takes the window at NS alignments (shift u: window position cb[u] at bit 0)
+// and, per guide, the mismatch masks M[u] of the guide at each alignment.  The
+// variants of the bulge family of shift u >= 1 (DNA or RNA bulge of one size,
+// every split j) count mm(j) = popc(left & bits[0,j)) + popc(right &
+// ~bits[0,j+gap)) with {left, right} = {M[u], M[0]}, so popc(M[u] & M[0]) - gap
+// is a lower bound of every mm(j) of the family: one AND and one popcount
+// filter the whole family.  Shift 0 alone is the no-bulge key (exact).  A
+// family that passes is bounded again on the two halves of its split range,
+// and only then are its splits counted one by one and emitted through the
+// family's variant entries (key index, threshold as bias; a key shared by
+// several variants is emitted by one of them).
+
+struct BulgeArgs {
+    DevGenome g;
+    PatternDesc pd;
+    int32_t cb[kMaxShifts];    // window position of mask bit 0 at each shift
+    uint32_t gap[kMaxShifts];  // RNA-bulge length of the family of each shift
+    uint32_t rev;              // reversed PAM: left part at shift 0, right part at shift u
+    uint32_t nunits;           // care classes
+    const uint32_t *table;
+    uint64_t limit;
+    uint64_t *hkey;
+    uint16_t *hmm;
+    unsigned long long *count;
+    unsigned long long *cand_count;
+    uint64_t cap;
+};
+
+// bits [0, x) of word w of an NW-word mask (x may be negative or past the end)
+__device__ __forceinline__ uint32_t low_bits(int32_t x, int w) {
+    const int32_t y = x - 32 * w;
+    return y <= 0 ? 0u : y >= 32 ? ~0u : (1u << y) - 1u;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t popc_below(const uint32_t (&m)[NW], int32_t x) {
+    uint32_t c = 0;
 #pragma unroll
-            for (int j = 0; j < kWordsPerThread; j++) {
-                const uint32_t wi = (uint32_t)(kWordsPerThread * t + j);
-                for (uint64_t m = (uint64_t)fwd[j] | ((uint64_t)rev[j] << 32); m; m &= m - 1, idx++) {
-                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                    if (all || (idx >= pass && idx < pass + cnt))
-                        s_list[idx - pass] = (uint16_t)(((wi * 32 + (b & 31)) << 1) | (b >> 5));
-                }
-            }
-        }
-        __syncthreads();
-        // each lane scores W candidates (register blocking); chunks of 64*W
-        // candidates go round-robin over the four waves
-        constexpr int W = COF_FUSED_WINDOWS;
-        for (uint32_t base = (uint32_t)wave * 64 * W; base < cnt; base += kThreads * W) {
-            const uint32_t i0 = base + lane;
-            if (i0 < cnt) {
-                uint32_t l[W][NW], h[W][NW], o[W][NW];
-                HitSink sink[W];
-                bool valid[W];
+    for (int w = 0; w < NW; w++) c += (uint32_t)__popc(m[w] & low_bits(x, w));
+    return c;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t popc_from(const uint32_t (&m)[NW], int32_t x) {
+    uint32_t c = 0;
 #pragma unroll
-                for (int w = 0; w < W; w++) {
-                    const uint32_t iw = i0 + 64 * w;
-                    valid[w] = iw < cnt;
-                    const uint32_t e = s_list[valid[w] ? iw : i0];  // an empty slot repeats candidate i0
-                    const uint32_t ls = e >> 1, strand = e & 1;
-                    const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
-                                                    : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase + 32 * kHaloL;
-                    window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
-                    sink[w] = HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase};
-                }
-                score_table<NW, GEN, W>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
-            }
-        }
-        __syncthreads();
+    for (int w = 0; w < NW; w++) c += (uint32_t)__popc(m[w] & ~low_bits(x, w));
+    return c;
+}
+
+// The variants of one family for lanes whose filter passed.  vh = the
+// family's [first split, split count, entry offset]; bias = the family filter
+// bias -(max threshold + gap + 1).
+template <int NW>
+__device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t vh, const uint32_t (&left)[NW],
+                                             const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
+                                             const HitSink &sink) {
+    const int32_t j0 = (int32_t)vh[0];
+    const uint32_t n = vh[1];
+    kptr_t ent = tab + vh[2];
+    uint32_t both[NW];
+#pragma unroll
+    for (int w = 0; w < NW; w++) both[w] = left[w] & right[w];
+    // splits j <= jm: positions >= jm + gap count `right`; the rest at least `both`
+    // splits j >= jm: positions < jm count `left`; the rest at least `both`
+    const int32_t jm = j0 + (int32_t)(n >> 1);
+    const int32_t b1 = (int32_t)(bias + popc_below<NW>(both, jm + (int32_t)gap) + popc_from<NW>(right, jm + (int32_t)gap));
+    const int32_t b2 = (int32_t)(bias + popc_below<NW>(left, jm) + popc_from<NW>(both, jm));
+    if ((b1 | b2) >= 0) return;  // neither half can hold a hit
+    for (uint32_t k = 0; k < n; k++) {
+        const int32_t j = j0 + (int32_t)k;
+        const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
+        if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
     }
+}
+
+template <int NW, bool GEN, int NS>
+__global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
+    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint16_t s_list[kListCap];
+    __shared__ uint32_t s_wsum[4];
+    constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS);
+    const uint32_t P = a.pd.P;
+    tile_candidates<1>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
+                       [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
+        const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
+        uint32_t l[NS][NW], h[NS][NW], o[NS][NW];
+#pragma unroll
+        for (int u = 0; u < NS; u++) {
+            const int64_t bit = strand == 0 ? (int64_t)ls + a.cb[u] + 32 * kHaloL
+                                            : (int64_t)ls + P - 32 * NW - (int64_t)a.cb[u] + 32 * kHaloL;
+            window_planes<NW>(rd, bit, strand, l[u], h[u], o[u]);
+        }
+        HitSink sink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u};
+        uint32_t cls[GEN ? NS : 1][5][NW];
+        if (GEN) {
+#pragma unroll
+            for (int u = 0; u < NS; u++)
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    cls[u][kClsT][w] = ~(l[u][w] | h[u][w] | o[u][w]);
+                    cls[u][kClsC][w] = l[u][w] & ~h[u][w];
+                    cls[u][kClsA][w] = h[u][w] & ~l[u][w];
+                    cls[u][kClsG][w] = l[u][w] & h[u][w];
+                    cls[u][kClsO][w] = o[u][w];
+                }
+        }
+        kptr_t kp = (kptr_t)a.table;
+        for (uint32_t c = 0; c < a.nunits; c++) {
+            uint32_t lc[NS][NW], hc[NS][NW], oc[NS][NW];
+            if (!GEN) {
+#pragma unroll
+                for (int w = 0; w < NW; w++) {
+                    const uint32_t care = kp[w];
+#pragma unroll
+                    for (int u = 0; u < NS; u++) {
+                        lc[u][w] = l[u][w] & care;
+                        hc[u][w] = h[u][w] & care;
+                        oc[u][w] = o[u][w] & care;
+                    }
+                }
+            }
+            const uint32_t n = kp[NW];
+            kp += kClassHdr;
+            for (uint32_t r = 0; r < n; r++, kp += S) {
+                uint32_t M[NS][NW];
+#pragma unroll
+                for (int u = 0; u < NS; u++)
+#pragma unroll
+                    for (int w = 0; w < NW; w++) {
+                        if (GEN) {
+                            uint32_t x = cls[u][kClsO][w] & kp[kClsO * NW + w];
+                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsG][w], kp[kClsG * NW + w], x, kAndOr);
+                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsA][w], kp[kClsA * NW + w], x, kAndOr);
+                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsC][w], kp[kClsC * NW + w], x, kAndOr);
+                            M[u][w] = __builtin_amdgcn_bitop3_b32(cls[u][kClsT][w], kp[kClsT * NW + w], x, kAndOr);
+                        } else {
+                            const uint32_t x = __builtin_amdgcn_bitop3_b32(hc[u][w], kp[NW + w], oc[u][w], kXorOr);
+                            M[u][w] = __builtin_amdgcn_bitop3_b32(lc[u][w], kp[w], x, kXorOr);
+                        }
+                    }
+                int32_t f[NS];
+                {
+                    uint32_t s = kp[KP + 1];
+#pragma unroll
+                    for (int w = 0; w < NW; w++) s += (uint32_t)__popc(M[0][w]);
+                    f[0] = (int32_t)s;
+                }
+                int32_t acc = f[0];
+#pragma unroll
+                for (int u = 1; u < NS; u++) {
+                    uint32_t s = kp[KP + 1 + u];
+#pragma unroll
+                    for (int w = 0; w < NW; w++) s += (uint32_t)__popc(M[u][w] & M[0][w]);
+                    f[u] = (int32_t)s;
+                    acc |= f[u];
+                }
+                if (acc < 0) {
+                    const kptr_t vb = (kptr_t)a.table + kp[KP];
+                    sink.gbase = vb[0];
+#pragma unroll
+                    for (int u = 0; u < NS; u++) {
+                        if (f[u] < 0) {
+                            const bool lft0 = u == 0 || a.rev;
+                            uint32_t left[NW], right[NW];
+#pragma unroll
+                            for (int w = 0; w < NW; w++) {
+                                left[w] = lft0 ? M[0][w] : M[u][w];
+                                right[w] = lft0 ? M[u][w] : M[0][w];
+                            }
+                            bulge_family<NW>((kptr_t)a.table, vb + 1 + 3 * u, left, right, a.gap[u],
+                                             kp[KP + 1 + u], sink);
+                        }
+                    }
+                }
+            }
+        }
+    });
 }
 
 // ------------------------------------------------------------------ launchers
@@ -629,6 +829,46 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
         else hipLaunchKernelGGL((fused_kernel<2, false>), grid, block, 0, s, a);
     }
     return hipGetLastError();
+}
+
+template <int NW, bool GEN>
+static hipError_t launch_bulge_ns(int ns, dim3 grid, const BulgeArgs &a, hipStream_t s) {
+    switch (ns) {
+        case 2: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 2>), grid, dim3(kThreads), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 3>), grid, dim3(kThreads), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 4>), grid, dim3(kThreads), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 5>), grid, dim3(kThreads), 0, s, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
+                        const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
+                        uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
+                        hipStream_t s) {
+    const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
+    if (limit == 0 || nunits == 0) return hipSuccess;
+    if (ns < 2 || ns > kMaxShifts || (nw != 1 && nw != 2)) return hipErrorInvalidValue;
+    BulgeArgs a{};
+    a.g = g;
+    a.pd = pd;
+    for (int u = 0; u < ns; u++) {
+        a.cb[u] = cb[u];
+        a.gap[u] = gap[u];
+    }
+    a.rev = rev ? 1u : 0u;
+    a.nunits = nunits;
+    a.table = table;
+    a.limit = limit;
+    a.hkey = hkey;
+    a.hmm = hmm;
+    a.count = count;
+    a.cand_count = cand_count;
+    a.cap = cap;
+    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos));
+    if (nw == 1) return gen ? launch_bulge_ns<1, true>(ns, grid, a, s) : launch_bulge_ns<1, false>(ns, grid, a, s);
+    return gen ? launch_bulge_ns<2, true>(ns, grid, a, s) : launch_bulge_ns<2, false>(ns, grid, a, s);
 }
 
 // Radix sort of the hit records by their 64-bit key (bits [0, end_bit)).

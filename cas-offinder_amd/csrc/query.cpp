@@ -134,6 +134,43 @@ std::string substr_checked(const std::string &s, uint64_t pos, uint64_t n = ~0ul
     return s.substr((size_t)pos, (size_t)std::min<uint64_t>(n, s.size() - pos));
 }
 
+// Every compare key a guide expands to, in the reference's order
+// (cas-offinder.cpp:647-682), with the bulge family it belongs to:
+// emit(key, info, shift, split) with shift 0 for the D = 0 key, i for a DNA
+// bulge of size i and D + i for an RNA bulge of size i, and split = j.
+template <typename Emit>
+void expand_guide(const std::string &g, uint32_t D, uint32_t R, bool rev, Emit &&emit) {
+    // first / last non-N index as std::string::find_*_not_of returns them
+    const uint64_t f = g.find_first_not_of('N'), l = g.find_last_not_of('N');
+    if (D == 0) {
+        emit(g, BulgeInfo{"0", 0}, 0u, 0u);
+    } else {
+        const std::string base = rev ? g + std::string(D, 'N') : std::string(D, 'N') + g;
+        for (uint32_t i = 1; i <= D; i++) {
+            const uint64_t lim = l + 2;  // wraps to 1 for an all-N guide
+            for (uint32_t j = (uint32_t)f; (uint64_t)j < lim; j++) {
+                std::string key = rev ? g.substr(0, j) + std::string(i, 'N') + substr_checked(g, j) +
+                                            std::string(D - i, 'N')
+                                      : std::string(D - i, 'N') + g.substr(0, j) + std::string(i, 'N') +
+                                            substr_checked(g, j);
+                const int idx = rev ? -(int)j : (int)j;
+                const bool is_base = key == base;
+                emit(key, BulgeInfo{is_base ? "0" : std::to_string(i), idx}, i, j);
+            }
+        }
+    }
+    for (uint32_t i = 1; i <= R; i++) {
+        const uint64_t lim = l + 1 - i;  // size_t arithmetic, may wrap
+        for (uint32_t j = (uint32_t)f; (uint64_t)j < lim; j++) {
+            std::string bulge = substr_checked(g, j, i);
+            std::string rest = substr_checked(g, (uint64_t)j + i);
+            std::string key = rev ? g.substr(0, j) + rest + std::string(D + i, 'N')
+                                  : std::string(D + i, 'N') + g.substr(0, j) + rest;
+            emit(key, BulgeInfo{bulge, rev ? -(int)j : (int)j}, D + i, j);
+        }
+    }
+}
+
 }  // namespace
 
 int parse_query(const char *text, size_t len, Query &q, std::string &err) {
@@ -153,6 +190,7 @@ int parse_query(const char *text, size_t len, Query &q, std::string &err) {
             q.R = (uint32_t)r;
         }
         const bool rev = tk[0][0] != 'N';  // tested before uppercasing (cas-offinder.cpp:619)
+        q.rev = rev;
         q.pattern = rev ? tk[0] + std::string(q.D, 'N') : std::string(q.D, 'N') + tk[0];
         upper(q.pattern);
         q.P = (uint32_t)q.pattern.size();
@@ -169,34 +207,10 @@ int parse_query(const char *text, size_t len, Query &q, std::string &err) {
             upper(g);
             const uint16_t thr = (uint16_t)atoi(t[1].c_str());
             const std::string id = t.size() == 3 ? t[2] : std::to_string(linecnt);
-            // first / last non-N index as std::string::find_*_not_of returns them
-            const uint64_t f = g.find_first_not_of('N'), l = g.find_last_not_of('N');
-            if (D == 0) {
-                add_key(m, g, id, thr, {"0", 0});
-            } else {
-                const std::string base = rev ? g + std::string(D, 'N') : std::string(D, 'N') + g;
-                for (uint32_t i = 1; i <= D; i++) {
-                    const uint64_t lim = l + 2;  // wraps to 1 for an all-N guide
-                    for (uint32_t j = (uint32_t)f; (uint64_t)j < lim; j++) {
-                        std::string key = rev ? g.substr(0, j) + std::string(i, 'N') + substr_checked(g, j) +
-                                                    std::string(D - i, 'N')
-                                              : std::string(D - i, 'N') + g.substr(0, j) + std::string(i, 'N') +
-                                                    substr_checked(g, j);
-                        const int idx = rev ? -(int)j : (int)j;
-                        add_key(m, key, id, thr, {key == base ? "0" : std::to_string(i), idx});
-                    }
-                }
-            }
-            for (uint32_t i = 1; i <= R; i++) {
-                const uint64_t lim = l + 1 - i;  // size_t arithmetic, may wrap
-                for (uint32_t j = (uint32_t)f; (uint64_t)j < lim; j++) {
-                    std::string bulge = substr_checked(g, j, i);
-                    std::string rest = substr_checked(g, (uint64_t)j + i);
-                    std::string key = rev ? g.substr(0, j) + rest + std::string(D + i, 'N')
-                                          : std::string(D + i, 'N') + g.substr(0, j) + rest;
-                    add_key(m, key, id, thr, {bulge, rev ? -(int)j : (int)j});
-                }
-            }
+            expand_guide(g, D, R, rev, [&](const std::string &key, BulgeInfo &&bi, uint32_t, uint32_t) {
+                add_key(m, key, id, thr, std::move(bi));
+            });
+            q.guide_lines.push_back(GuideLine{g, id, thr});
             if (entrycnt == 0) entrycnt = t.size();
             else if (entrycnt != t.size())
                 throw ParseError{"The number of entries below 2nd line should be consistent."};
@@ -296,6 +310,184 @@ int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, st
     return 0;
 }
 
+namespace {
+
+// Bulge families of every guide of q (see BulgeGuide).  Each compare key is
+// emitted by exactly one variant, its first in expansion order, with the key's
+// (first-wins) threshold; other variants of the same key carry kNeverHit.
+// Leaves q.bulge empty when the query cannot use device-side expansion.
+void encode_bulge(Query &q) {
+    q.bulge.clear();
+    const uint32_t D = q.D, R = q.R;
+    if (D + R == 0 || D + R + 1 > (uint32_t)kMaxShifts) return;
+    std::vector<char> emitted(q.keys.size(), 0);
+    bool ok = true;
+    for (const GuideLine &gl : q.guide_lines) {
+        const std::string &g = gl.seq;
+        if (g.size() > 64) { ok = false; break; }
+        BulgeGuide bg;
+        for (size_t k = 0; k < g.size(); k++) {
+            const uint32_t mm = mismatch_classes(g[k]);
+            if (!mm) continue;
+            const uint64_t bit = 1ull << k;
+            bg.care |= bit;
+            for (int c = 0; c < 5; c++)
+                if (mm & (1u << c)) bg.mask[c] |= bit;
+            const char b = g[k];
+            if (b == 'A' || b == 'C' || b == 'G' || b == 'T') {
+                const uint32_t code = b == 'T' ? 0 : b == 'C' ? 1 : b == 'A' ? 2 : 3;
+                if (code & 1) bg.lo |= bit;
+                if (code & 2) bg.hi |= bit;
+            } else {
+                bg.concrete = false;
+            }
+        }
+        std::vector<std::vector<uint32_t>> fam(D + R + 1);
+        for (auto &t : bg.tmax) t = -1;
+        bool any = false;
+        expand_guide(g, D, R, q.rev, [&](const std::string &key, BulgeInfo &&, uint32_t u, uint32_t j) {
+            if (fam[u].empty()) bg.j0[u] = (int32_t)j;
+            else if ((int64_t)j != bg.j0[u] + (int64_t)fam[u].size() / 2) ok = false;  // splits must be consecutive
+            auto it = std::lower_bound(q.keys.begin(), q.keys.end(), key,
+                                       [](const CompareKey &a, const std::string &b) { return a.seq < b; });
+            if (it == q.keys.end() || it->seq != key) { ok = false; return; }
+            const size_t ki = (size_t)(it - q.keys.begin());
+            uint32_t bias = kNeverHit;
+            if (!emitted[ki]) {
+                emitted[ki] = 1;
+                bias = (uint32_t)(-(int64_t)it->threshold - 1);
+                bg.tmax[u] = std::max<int32_t>(bg.tmax[u], it->threshold);
+                any = true;
+            }
+            fam[u].push_back(bias);
+            fam[u].push_back((uint32_t)ki);
+        });
+        if (!ok) break;
+        if (!any) continue;  // every key of this guide is emitted by an earlier one
+        for (uint32_t u = 0; u <= D + R; u++) {
+            bg.n[u] = (uint32_t)(fam[u].size() / 2);
+            bg.ent.insert(bg.ent.end(), fam[u].begin(), fam[u].end());
+        }
+        q.bulge.push_back(std::move(bg));
+    }
+    if (!ok) q.bulge.clear();
+}
+
+}  // namespace
+
+int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
+    t = BulgeTable();
+    if (qs.empty()) return 0;
+    const Query &q0 = *qs[0].q;
+    const uint32_t D = q0.D, R = q0.R;
+    const int ns = (int)(D + R + 1);
+    if (D + R == 0 || ns > kMaxShifts) return 0;
+    uint64_t allcare = 0;
+    bool gen = false;
+    for (const BulgeRef &r : qs) {
+        const Query &q = *r.q;
+        if (q.D != D || q.R != R || q.rev != q0.rev || q.pattern != q0.pattern) return 0;
+        if (q.bulge.empty() && !q.keys.empty()) return 0;
+        for (const BulgeGuide &bg : q.bulge) {
+            allcare |= bg.care;
+            gen |= !bg.concrete;
+        }
+    }
+    const int gcb = allcare ? __builtin_ctzll(allcare) : 0;
+    const int top = allcare ? 63 - __builtin_clzll(allcare) : 0;
+    const int nw = top - gcb + 1 <= 32 ? 1 : 2;
+    // window alignment of each shift: the left or right part of a variant
+    // moves by the bulge size (cas-offinder.cpp:650-682 key layouts)
+    const int sx = q0.rev ? 0 : (int)D;
+    for (int u = 0; u < ns; u++) {
+        int delta = 0;
+        if (u >= 1 && u <= (int)D) delta = q0.rev ? u : -u;                  // DNA bulge of size u
+        else if (u > (int)D) delta = q0.rev ? -(u - (int)D) : u - (int)D;    // RNA bulge of size u - D
+        t.cb[u] = sx + delta + gcb;
+        t.gap[u] = u > (int)D ? (uint32_t)(u - (int)D) : 0u;
+        // every LDS word the kernel reads for a window start in the tile stays
+        // inside the staged tile + halo, on both strands
+        const int64_t P = q0.P, halo = 32 * kHaloL;
+        const int64_t plus_lo = t.cb[u] + halo, plus_hi = kTilePos - 1 + t.cb[u] + halo + 32 * (nw - 1);
+        const int64_t minus_lo = P - 32 * nw - t.cb[u] + halo;
+        const int64_t minus_hi = kTilePos - 1 + P - 32 * nw - t.cb[u] + halo + 32 * (nw - 1);
+        if (plus_lo < 0 || minus_lo < 0 || (plus_hi >> 5) + 1 >= kLdsWords || (minus_hi >> 5) + 1 >= kLdsWords)
+            return 0;
+    }
+    t.nw = nw;
+    t.gen = gen;
+    t.ns = ns;
+    t.rev = q0.rev;
+    t.pat = q0.pat;
+    auto word = [&](uint64_t x, int w) { return (uint32_t)((x >> gcb) >> (32 * w)); };
+    // care classes (one class of all-ones care for general tables)
+    struct Item {
+        const BulgeGuide *bg;
+        uint32_t gbase;
+    };
+    std::map<std::vector<uint32_t>, std::vector<Item>> cls;
+    std::vector<std::vector<uint32_t>> order;
+    for (const BulgeRef &r : qs)
+        for (const BulgeGuide &bg : r.q->bulge) {
+            std::vector<uint32_t> c(nw, ~0u);
+            if (!gen)
+                for (int w = 0; w < nw; w++) c[w] = word(bg.care, w);
+            auto it = cls.find(c);
+            if (it == cls.end()) { order.push_back(c); cls[c].push_back({&bg, r.gbase}); }
+            else it->second.push_back({&bg, r.gbase});
+        }
+    const int KP = bulge_planes(nw, gen), S = bulge_stride(nw, gen, ns);
+    size_t rec_end = 0;
+    for (auto &c : order) rec_end += kClassHdr + cls[c].size() * (size_t)S;
+    std::vector<uint32_t> &out = t.words;
+    out.reserve(rec_end * 2);
+    out.resize(rec_end, 0);
+    size_t p = 0;
+    for (auto &c : order) {
+        const auto &items = cls[c];
+        for (int w = 0; w < nw; w++) out[p + w] = c[w];
+        out[p + nw] = (uint32_t)items.size();
+        p += kClassHdr;
+        for (const Item &it : items) {
+            const BulgeGuide &bg = *it.bg;
+            uint32_t *rec = &out[p];
+            if (gen) {
+                for (int c5 = 0; c5 < 5; c5++)
+                    for (int w = 0; w < nw; w++) rec[c5 * nw + w] = word(bg.mask[c5], w);
+            } else {
+                for (int w = 0; w < nw; w++) {
+                    rec[w] = word(bg.lo, w);
+                    rec[nw + w] = word(bg.hi, w);
+                }
+            }
+            for (int u = 0; u < ns; u++)
+                rec[KP + 1 + u] = bg.tmax[u] < 0 ? kNeverHit : (uint32_t)(-(int64_t)bg.tmax[u] - (int64_t)t.gap[u] - 1);
+            // variant block
+            const size_t vb = out.size();
+            rec[KP] = (uint32_t)vb;  // (rec may move below; written before the resize)
+            out.resize(vb + 1 + 3 * (size_t)ns);
+            out[vb] = it.gbase;
+            size_t e = out.size();
+            size_t src = 0;
+            for (int u = 0; u < ns; u++) {
+                out[vb + 1 + 3 * u] = (uint32_t)(bg.j0[u] - gcb);
+                out[vb + 2 + 3 * u] = bg.n[u];
+                out[vb + 3 + 3 * u] = (uint32_t)(e + src);
+                src += 2 * (size_t)bg.n[u];
+            }
+            out.insert(out.end(), bg.ent.begin(), bg.ent.end());
+            p += S;
+        }
+    }
+    t.units = (uint32_t)order.size();
+    for (const BulgeRef &r : qs) {
+        t.nguides += r.q->bulge.size();
+        t.nkeys += r.q->keys.size();
+        t.care_sum += r.q->care_sum;
+    }
+    return 1;
+}
+
 int encode_query(Query &q, std::string &err) {
     const uint32_t P = q.P;
     if (P == 0) { err = "Empty pattern."; return -1; }
@@ -333,6 +525,7 @@ int encode_query(Query &q, std::string &err) {
         uint32_t units;
         encode_records({refs[n]}, P, q.cbase, q.nw, q.gen, false, q.single_words, units);
     }
+    encode_bulge(q);
     return 0;
 }
 

@@ -23,11 +23,39 @@ struct CompareKey {
     std::vector<BulgeInfo> infos;
 };
 
+struct GuideLine {
+    std::string seq;  // uppercase
+    std::string id;
+    uint16_t threshold = 0;
+};
+
+// A guide with every DNA/RNA-bulge variant it expands to, for device-side
+// expansion.  Shift 0 is the guide's own alignment (the no-bulge key when D = 0);
+// shift i (1..D) is the family of DNA bulges of size i and shift D + i
+// (1..R) the family of RNA bulges of size i.  A variant with split j counts
+//   mm(j) = popcount(left & bits[0, j)) + popcount(right & ~bits[0, j + gap))
+// over the mismatch masks of the guide at two alignments (bit k <-> guide
+// position k): the left part at the family's shift and the right part at
+// shift 0 for a forward PAM, the other way round for a reversed PAM; gap is the
+// RNA-bulge length (0 for DNA bulges).
+struct BulgeGuide {
+    uint64_t care = 0, lo = 0, hi = 0;  // positions that can mismatch; 2-bit code planes there
+    uint64_t mask[5] = {};              // per class: positions where that class mismatches
+    bool concrete = true;               // every care position is A/C/G/T
+    int32_t j0[kMaxShifts] = {};        // first split of each family (guide position)
+    uint32_t n[kMaxShifts] = {};        // splits of each family
+    int32_t tmax[kMaxShifts] = {};      // largest threshold among emitting variants, -1 if none
+    std::vector<uint32_t> ent;          // per family, per split: bias, query-local key index
+};
+
 struct Query {
     std::string chrdir;
     std::string pattern;  // padded, uppercase
     uint32_t P = 0, D = 0, R = 0;
+    bool rev = false;                   // reversed (5') PAM: pattern line does not start with N
     std::vector<CompareKey> keys;  // bytewise key order (std::map order)
+    std::vector<GuideLine> guide_lines;
+    std::vector<BulgeGuide> bulge;      // bulge families of the guides (D + R > 0, D + R < kMaxShifts)
 
     // device encodings (encode_query)
     PatternDesc pat{};
@@ -70,5 +98,25 @@ struct KeyRef {
     uint32_t index;  // written into the record; hits report it
 };
 int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, std::string &err);
+
+// Bulge-family table of queries sharing padded pattern, D, R and PAM side.
+struct BulgeTable {
+    std::vector<uint32_t> words;
+    uint32_t units = 0;  // care classes
+    int nw = 1;
+    bool gen = false;
+    int ns = 0;                      // shifts per guide
+    int32_t cb[kMaxShifts] = {};     // window position of bit 0 at each shift
+    uint32_t gap[kMaxShifts] = {};   // RNA-bulge length of each shift's family
+    bool rev = false;
+    PatternDesc pat{};  // finder pattern of the group
+    uint64_t nguides = 0, nkeys = 0, care_sum = 0;
+};
+struct BulgeRef {
+    const Query *q;
+    uint32_t gbase;  // global index of the query's first key
+};
+// 1: built; 0: the queries do not fit the bulge-family kernel (use key tables).
+int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t);
 
 }  // namespace cof

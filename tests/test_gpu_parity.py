@@ -156,3 +156,72 @@ def test_batched_multi_query_matches_single(ctx, tmp_path):
     for i, t in enumerate(texts):
         exp = H.record_lines(H.oracle_records(tmp_path, t, name="q%d.txt" % i))
         assert H.record_lines(rb.format(i)) == exp
+
+
+def _planted_guides(r, recs, pattern, core, n, iupac=False):
+    """Guides copied from windows that pass the PAM (either strand), with a
+    planted DNA bulge (a guide base deleted), RNA bulge (a base inserted) and
+    up to two substitutions, so that the bulge families have sites to find."""
+    P = len(pattern)
+    rev = pattern[0] != "N"
+    rpat = H.revcomp(pattern)
+    out = []
+    while len(out) < n:
+        _, s = r.choice(recs)
+        x = r.randrange(0, len(s) - P)
+        w = s[x:x + P]
+        if H.window_passes(rpat, w):
+            w = H.revcomp(w)
+        elif not H.window_passes(pattern, w):
+            continue
+        c = list(w[P - core:] if rev else w[:core])
+        kind = r.choice(["dna", "rna", "none"])
+        if kind == "dna":  # the genome holds one base more than the guide
+            del c[r.randrange(1, core - 1)]
+            c = c + [r.choice("ACGT")] if rev else [r.choice("ACGT")] + c
+        elif kind == "rna":  # the guide holds one base more than the genome
+            c.insert(r.randrange(1, core - 1), r.choice("ACGT"))
+            c = c[:core] if rev else c[1:]
+        for _ in range(r.randint(0, 2)):
+            c[r.randrange(core)] = r.choice("ACGT")
+        if iupac and r.random() < 0.5:
+            c[r.randrange(core)] = r.choice("RYKMSWBDHVN")
+        out.append("".join(ch if ch in "ACGTRYKMSWBDHVN" else "A" for ch in c))
+    return out
+
+
+BULGE_CASES = [
+    # (pattern, guide core length, D, R, IUPAC codes in guides)
+    ("N" * 21 + "RG", 20, 1, 0, False),
+    ("N" * 21 + "RG", 20, 0, 1, False),
+    ("N" * 21 + "RG", 20, 1, 1, False),
+    ("N" * 21 + "RG", 20, 2, 1, True),
+    ("N" * 21 + "RG", 20, 2, 2, False),
+    ("N" * 21 + "RG", 20, 3, 1, False),
+    ("N" * 21 + "RG", 20, 4, 1, False),     # more shifts than the bulge kernel takes: key tables
+    ("TTTV" + "N" * 23, 23, 1, 1, False),
+    ("TTTV" + "N" * 23, 23, 2, 0, True),
+    ("TTTV" + "N" * 23, 23, 0, 2, False),
+    ("N" * 40 + "NGG", 40, 1, 1, False),    # guide span > 32: two-word masks
+    ("N" * 20 + "NNAGAAW", 20, 1, 1, True),
+]
+
+
+@pytest.mark.parametrize("case", range(len(BULGE_CASES)))
+def test_bulge_families(ctx, tmp_path, case):
+    """Batched (device-side bulge expansion) and plain (one launch per key)
+    against the oracle on planted DNA/RNA-bulge sites."""
+    pattern, core, D, R, iupac = BULGE_CASES[case]
+    r = random.Random(100 + case)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(20000, 60000), "ACGT", n_runs=r.randint(0, 3))) for i in range(4)]
+    gp = str(tmp_path / "b.fa")
+    H.write_fasta(gp, recs)
+    rev = pattern[0] != "N"
+    tail = "N" * (len(pattern) - core)
+    cores = _planted_guides(r, recs, pattern, core, 6, iupac)
+    guides = [tail + c if rev else c + tail for c in cores]
+    guides.append(guides[0])  # duplicate line: its keys print under the first id
+    lines = ["%s %d %d" % (pattern, D, R)] + ["%s %d g%d" % (g, r.randint(1, 5), i) for i, g in enumerate(guides)]
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "bulge case %d" % case)
+    assert len(exp) > 0
