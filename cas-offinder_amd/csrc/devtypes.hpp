@@ -47,12 +47,18 @@ constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 // Records, grouped by care class like concrete key tables ([care[NW], nguides,
 // 0, 0] headers), are [guide planes][variant block offset][bias per shift],
 // padded to 4 dwords; planes are glo[NW], ghi[NW] (concrete) or the five class
-// masks [class * NW + word] (general).  A variant block is [gbase, then per
-// shift: first split (bit coordinate, int32), split count, entry offset] and
-// each entry is [bias, query-local key index].
+// masks [class * NW + word] (general).  A variant block is [gbase, 0, 0, 0]
+// followed by one family block per shift: [first split (bit coordinate,
+// int32), split count, entry offset, segment count] and kBulgeSegs segment
+// records [first split index, split count, below[NW], inner[NW], above[NW]]
+// (the masks of the segment's lower bound), then the entries [bias,
+// query-local key index] of every split.
 constexpr int kMaxShifts = 5;
+constexpr int kBulgeSegs = 4;
 constexpr int bulge_planes(int nw, bool gen) { return gen ? 5 * nw : 2 * nw; }
 constexpr int bulge_stride(int nw, bool gen, int ns) { return (bulge_planes(nw, gen) + 1 + ns + 3) & ~3; }
+constexpr int bulge_seg_words(int nw) { return 2 + 3 * nw; }
+constexpr int bulge_family_words(int nw) { return 4 + kBulgeSegs * bulge_seg_words(nw); }
 constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that never hits
 
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.

@@ -174,6 +174,54 @@ __device__ __forceinline__ void emit_hit(const HitSink &o, uint32_t key, uint32_
     }
 }
 
+// Hits of a workgroup are staged in LDS and written out once per candidate
+// pass with one global atomic, so the hit counter sees one atomic per tile
+// pass instead of one per hit (same-address atomics serialize in one L2
+// channel).  An entry packs global key index (bits 23..46), mismatches
+// (16..22), strand (15) and the start's offset in the tile (0..14).
+constexpr int kStageCap = 768;
+struct HitStage {
+    uint64_t ent[kStageCap];
+    uint32_t n;  // entries appended this pass; past kStageCap they went to HBM directly
+    uint32_t pad;
+    unsigned long long base;
+};
+static_assert(kTilePos <= (1 << 15) && kMaxPattern < (1 << 7), "stage entry fields");
+
+struct TileSink {
+    HitSink g;      // direct path when the stage is full
+    HitStage *st;
+    uint32_t low;   // strand << 15 | tile offset
+};
+
+__device__ __forceinline__ void emit_hit(const TileSink &o, uint32_t key, uint32_t mm) {
+    const uint32_t i = atomicAdd(&o.st->n, 1u);
+    if (i < (uint32_t)kStageCap)
+        o.st->ent[i] = ((uint64_t)(o.g.gbase + key) << 23) | (mm << 16) | o.low;
+    else
+        emit_hit(o.g, key, mm);
+}
+
+// Writes the staged hits of the pass (all threads, after a barrier) and
+// empties the stage.
+__device__ __forceinline__ void flush_stage(HitStage *st, uint64_t t0, uint64_t *hkey, uint16_t *hmm,
+                                            unsigned long long *count, uint64_t cap) {
+    const uint32_t n = min(st->n, (uint32_t)kStageCap);
+    if (threadIdx.x == 0) st->base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
+    __syncthreads();
+    const unsigned long long base = st->base;
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+        const uint64_t e = st->ent[i];
+        const unsigned long long o = base + i;
+        if (o < cap) {
+            hkey[o] = ((e >> 23) << kHitKeyShift) | ((t0 + (e & 0x7FFF)) << 1) | ((e >> 15) & 1);
+            hmm[o] = (uint16_t)((e >> 16) & 0x7F);
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) st->n = 0;
+}
+
 // NW words of the window planes starting at plane bit `bit` (rd(i) reads word
 // i of the three planes); for strand 1 the words are reverse complemented.
 // Results have lo = hi = 0 wherever ot = 1.
@@ -208,6 +256,7 @@ __device__ __forceinline__ void window_planes(Rd rd, int64_t bit, uint32_t stran
 // v_bitop3_b32 truth tables (bit index = a*4 + b*2 + c)
 constexpr uint32_t kXorOr = 0xBE;  // (a ^ b) | c
 constexpr uint32_t kAndOr = 0xEA;  // (a & b) | c
+constexpr uint32_t kAnd3 = 0x80;   // a & b & c
 
 // Concrete key table (every comparable key base is A/C/G/T): a run of care
 // classes, each a header [care[NW], nkeys, 0...] of kClassHdr dwords followed
@@ -270,8 +319,8 @@ __device__ __forceinline__ int32_t score_general(kptr_t kw, const uint32_t (&cls
 // Scores of 32 keys are OR-ed into one sign test; only a block in which some
 // lane has a hit is scored again key by key to emit.  Window w emits only if
 // valid[w] (an invalid slot duplicates a valid window of the same lane).
-template <int W, int S, int B, typename Score>
-__device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &score, const HitSink (&o)[W],
+template <int W, int S, int B, typename Score, typename Sink>
+__device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &score, const Sink (&o)[W],
                                            const bool (&valid)[W]) {
     for (uint32_t k = 0; k < n; k++, kp += S) {
 #pragma unroll
@@ -306,8 +355,8 @@ __device__ __forceinline__ bool block_hit(kptr_t kp, const Score &score) {
 // Tables of the fused kernel hold a multiple of 8 records per run (padded
 // with records that never hit), so only the plain path's one-key tables reach
 // the per-key tail.
-template <int W, int S, int B, typename Score>
-__device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const HitSink (&o)[W],
+template <int W, int S, int B, typename Score, typename Sink>
+__device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const Sink (&o)[W],
                                           const bool (&valid)[W]) {
     uint32_t k = 0;
     for (; k + kBlockKeys <= n; k += kBlockKeys, kp += kBlockKeys * S)
@@ -319,10 +368,10 @@ __device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &sc
 
 // Scores a whole key table (nunits = care classes for concrete tables, keys
 // for general ones) against W windows (planes l, h, o of each).
-template <int NW, bool GEN, int W>
+template <int NW, bool GEN, int W, typename Sink>
 __device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const uint32_t (&l)[W][NW],
                                             const uint32_t (&h)[W][NW], const uint32_t (&o)[W][NW],
-                                            const HitSink (&sink)[W], const bool (&valid)[W]) {
+                                            const Sink (&sink)[W], const bool (&valid)[W]) {
     constexpr int S = KeyFmt<NW, GEN>::stride, B = KeyFmt<NW, GEN>::bias;
     if (GEN) {
         uint32_t cls[W][5][NW];
@@ -485,13 +534,18 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 // an LDS list of kListCap entries per pass, then cands(t0, rd, entry, valid)
 // per lane for chunks of 64 * W candidates (round-robin over the four waves).
 // Entries are tile offset << 1 | strand (forward entries of a word first).
+// Hits go through the workgroup's LDS stage st, flushed to (hkey, hmm,
+// count, cap) after every pass.
 template <int W, typename Cands>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 unsigned long long *cand_count, uint32_t *s_lo, uint32_t *s_hi,
-                                                uint32_t *s_ot, uint16_t *s_list, uint32_t *s_wsum, Cands &&cands) {
+                                                uint32_t *s_ot, uint16_t *s_list, uint32_t *s_wsum, HitStage *st,
+                                                uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                                                uint64_t cap, Cands &&cands) {
     const int t = threadIdx.x;
     const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
     stage_tile(g, w0, s_lo, s_hi, s_ot);
+    if (t == 0) st->n = 0;
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
     uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
@@ -551,6 +605,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
             }
         }
         __syncthreads();
+        flush_stage(st, t0, hkey, hmm, count, cap);
     }
 }
 
@@ -572,12 +627,13 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
+    __shared__ HitStage s_stage;
     constexpr int W = COF_FUSED_WINDOWS;  // candidates per lane (register blocking)
     const uint32_t P = a.pd.P;
-    tile_candidates<W>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
-                       [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
+    tile_candidates<W>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey, a.hmm,
+                       a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
                            uint32_t l[W][NW], h[W][NW], o[W][NW];
-                           HitSink sink[W];
+                           TileSink sink[W];
 #pragma unroll
                            for (int w = 0; w < W; w++) {
                                const uint32_t ls = e[w] >> 1, strand = e[w] & 1;
@@ -585,7 +641,8 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
                                                                : (int64_t)ls + P - 32 * NW - (int64_t)a.cbase +
                                                                      32 * kHaloL;
                                window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
-                               sink[w] = HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase};
+                               sink[w] = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase},
+                                                  &s_stage, (strand << 15) | ls};
                            }
                            score_table<NW, GEN, W>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
                        });
@@ -644,29 +701,36 @@ __device__ __forceinline__ uint32_t popc_from(const uint32_t (&m)[NW], int32_t x
     return c;
 }
 
-// The variants of one family for lanes whose filter passed.  vh = the
-// family's [first split, split count, entry offset]; bias = the family filter
-// bias -(max threshold + gap + 1).
+// The variants of one family for lanes whose filter passed.  fb = the
+// family block [first split, split count, entry offset, segments, segment
+// records]; bias = the family filter bias -(max threshold + gap + 1).  Each
+// segment of splits [ja, jz] is bounded by popc(left below ja) +
+// popc(left & right in [ja, jz + gap)) + popc(right from jz + gap) - gap;
+// only the splits of a segment that passes are counted exactly.
 template <int NW>
-__device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t vh, const uint32_t (&left)[NW],
+__device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32_t (&left)[NW],
                                              const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
-                                             const HitSink &sink) {
-    const int32_t j0 = (int32_t)vh[0];
-    const uint32_t n = vh[1];
-    kptr_t ent = tab + vh[2];
-    uint32_t both[NW];
+                                             const TileSink &sink) {
+    constexpr int SW = bulge_seg_words(NW);
+    const int32_t j0 = (int32_t)fb[0];
+    const uint32_t nseg = fb[3];
+    kptr_t ent = tab + fb[2];
+    for (uint32_t sg = 0; sg < nseg; sg++) {
+        const kptr_t sr = fb + 4 + sg * SW;
+        uint32_t b = bias;
 #pragma unroll
-    for (int w = 0; w < NW; w++) both[w] = left[w] & right[w];
-    // splits j <= jm: positions >= jm + gap count `right`; the rest at least `both`
-    // splits j >= jm: positions < jm count `left`; the rest at least `both`
-    const int32_t jm = j0 + (int32_t)(n >> 1);
-    const int32_t b1 = (int32_t)(bias + popc_below<NW>(both, jm + (int32_t)gap) + popc_from<NW>(right, jm + (int32_t)gap));
-    const int32_t b2 = (int32_t)(bias + popc_below<NW>(left, jm) + popc_from<NW>(both, jm));
-    if ((b1 | b2) >= 0) return;  // neither half can hold a hit
-    for (uint32_t k = 0; k < n; k++) {
-        const int32_t j = j0 + (int32_t)k;
-        const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
-        if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+        for (int w = 0; w < NW; w++) {
+            b += (uint32_t)__popc(left[w] & sr[2 + w]);
+            b += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(left[w], right[w], sr[2 + NW + w], kAnd3));
+            b += (uint32_t)__popc(right[w] & sr[2 + 2 * NW + w]);
+        }
+        if ((int32_t)b >= 0) continue;
+        const uint32_t a = sr[0], cnt = sr[1];
+        for (uint32_t k = a; k < a + cnt; k++) {
+            const int32_t j = j0 + (int32_t)k;
+            const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
+            if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+        }
     }
 }
 
@@ -675,10 +739,11 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
+    __shared__ HitStage s_stage;
     constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS);
     const uint32_t P = a.pd.P;
-    tile_candidates<1>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
-                       [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
+    tile_candidates<1>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey, a.hmm,
+                       a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
         const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
         uint32_t l[NS][NW], h[NS][NW], o[NS][NW];
 #pragma unroll
@@ -687,7 +752,7 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                                             : (int64_t)ls + P - 32 * NW - (int64_t)a.cb[u] + 32 * kHaloL;
             window_planes<NW>(rd, bit, strand, l[u], h[u], o[u]);
         }
-        HitSink sink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u};
+        TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, &s_stage, (strand << 15) | ls};
         uint32_t cls[GEN ? NS : 1][5][NW];
         if (GEN) {
 #pragma unroll
@@ -753,7 +818,8 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                 }
                 if (acc < 0) {
                     const kptr_t vb = (kptr_t)a.table + kp[KP];
-                    sink.gbase = vb[0];
+                    sink.g.gbase = vb[0];
+                    constexpr int FB = bulge_family_words(NW);
 #pragma unroll
                     for (int u = 0; u < NS; u++) {
                         if (f[u] < 0) {
@@ -764,7 +830,7 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                                 left[w] = lft0 ? M[0][w] : M[u][w];
                                 right[w] = lft0 ? M[u][w] : M[0][w];
                             }
-                            bulge_family<NW>((kptr_t)a.table, vb + 1 + 3 * u, left, right, a.gap[u],
+                            bulge_family<NW>((kptr_t)a.table, vb + 4 + FB * u, left, right, a.gap[u],
                                              kp[KP + 1 + u], sink);
                         }
                     }

@@ -375,6 +375,12 @@ void encode_bulge(Query &q) {
 
 }  // namespace
 
+// bits [0, x) of word w of a multi-word mask (x may be negative or past the end)
+static uint32_t low_mask(int32_t x, int w) {
+    const int32_t y = x - 32 * w;
+    return y <= 0 ? 0u : y >= 32 ? ~0u : (1u << y) - 1u;
+}
+
 int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
     t = BulgeTable();
     if (qs.empty()) return 0;
@@ -462,18 +468,38 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
             }
             for (int u = 0; u < ns; u++)
                 rec[KP + 1 + u] = bg.tmax[u] < 0 ? kNeverHit : (uint32_t)(-(int64_t)bg.tmax[u] - (int64_t)t.gap[u] - 1);
-            // variant block
+            // variant block (rec is not used past this point: out grows)
             const size_t vb = out.size();
-            rec[KP] = (uint32_t)vb;  // (rec may move below; written before the resize)
-            out.resize(vb + 1 + 3 * (size_t)ns);
+            rec[KP] = (uint32_t)vb;
+            const int FB = bulge_family_words(nw), SW = bulge_seg_words(nw);
+            out.resize(vb + 4 + (size_t)ns * FB, 0);
             out[vb] = it.gbase;
-            size_t e = out.size();
+            const size_t e = out.size();
             size_t src = 0;
             for (int u = 0; u < ns; u++) {
-                out[vb + 1 + 3 * u] = (uint32_t)(bg.j0[u] - gcb);
-                out[vb + 2 + 3 * u] = bg.n[u];
-                out[vb + 3 + 3 * u] = (uint32_t)(e + src);
-                src += 2 * (size_t)bg.n[u];
+                uint32_t *fb = &out[vb + 4 + (size_t)u * FB];
+                const int32_t jb0 = bg.j0[u] - gcb;
+                const uint32_t n = bg.n[u], nseg = std::min<uint32_t>(n, kBulgeSegs);
+                fb[0] = (uint32_t)jb0;
+                fb[1] = n;
+                fb[2] = (uint32_t)(e + src);
+                fb[3] = nseg;
+                // segment s holds splits [a, a + cnt); its lower bound counts
+                // `left` below the first split, `left & right` inside and
+                // `right` from the last split + gap on
+                for (uint32_t sg = 0; sg < nseg; sg++) {
+                    const uint32_t a = (uint32_t)((uint64_t)n * sg / nseg), b = (uint32_t)((uint64_t)n * (sg + 1) / nseg);
+                    uint32_t *sr = fb + 4 + sg * SW;
+                    sr[0] = a;
+                    sr[1] = b - a;
+                    const int32_t ja = jb0 + (int32_t)a, jz = jb0 + (int32_t)b - 1 + (int32_t)t.gap[u];
+                    for (int w = 0; w < nw; w++) {
+                        sr[2 + w] = low_mask(ja, w);
+                        sr[2 + nw + w] = low_mask(jz, w) & ~low_mask(ja, w);
+                        sr[2 + 2 * nw + w] = ~low_mask(jz, w);
+                    }
+                }
+                src += 2 * (size_t)n;
             }
             out.insert(out.end(), bg.ent.begin(), bg.ent.end());
             p += S;
