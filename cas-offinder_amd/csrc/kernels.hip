@@ -713,9 +713,11 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
                                              const TileSink &sink) {
     constexpr int SW = bulge_seg_words(NW);
     const int32_t j0 = (int32_t)fb[0];
-    const uint32_t nseg = fb[3];
     kptr_t ent = tab + fb[2];
-    for (uint32_t sg = 0; sg < nseg; sg++) {
+    // all kBulgeSegs records are read (unused ones have no splits), so their
+    // scalar loads issue together
+#pragma unroll
+    for (int sg = 0; sg < kBulgeSegs; sg++) {
         const kptr_t sr = fb + 4 + sg * SW;
         uint32_t b = bias;
 #pragma unroll
@@ -724,8 +726,8 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
             b += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(left[w], right[w], sr[2 + NW + w], kAnd3));
             b += (uint32_t)__popc(right[w] & sr[2 + 2 * NW + w]);
         }
-        if ((int32_t)b >= 0) continue;
         const uint32_t a = sr[0], cnt = sr[1];
+        if ((int32_t)b >= 0 || cnt == 0) continue;
         for (uint32_t k = a; k < a + cnt; k++) {
             const int32_t j = j0 + (int32_t)k;
             const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
