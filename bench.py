@@ -270,6 +270,16 @@ def main():
                      "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
                              "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
     }
+    if not args.no_plain and st_p["t_finder_ms"] > 0:
+        # plain path's finder (finder_count_kernel + tile-count scan +
+        # finder_write_kernel): algorithmic bytes = 0.375 B read per scanned
+        # position + a (locus u32, flag u8) entry written per locus with a PAM
+        fbytes = 0.375 * st_p["n_finder_positions"] + 5 * st_p["n_loci"]
+        fach = fbytes / (st_p["t_finder_ms"] * 1e-3) / 1e9
+        out["finder_roofline"] = {"kernel": "finder_count_kernel + finder_write_kernel (plain path)", "bound": "hbm",
+                                  "algorithmic_bytes_per_step": int(fbytes), "ms_per_step": round(st_p["t_finder_ms"], 4),
+                                  "achieved": round(fach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                                  "frac": round(fach / HBM_PEAK_GBS, 4)}
     if args.serve_requests > 0:
         log("serving: %d requests at %.1f QPS" % (args.serve_requests, args.serve_qps))
         out["serving"] = serve_bench(ctx, args.serve_requests, args.serve_qps)

@@ -49,6 +49,8 @@ class Stats(C.Structure):
         ("n_base_cmp", C.c_uint64),
         ("n_model_ops", C.c_uint64),
         ("t_collect_ms", C.c_double),
+        ("n_loci", C.c_uint64),
+        ("n_finder_positions", C.c_uint64),
     ]
 
     def as_dict(self):

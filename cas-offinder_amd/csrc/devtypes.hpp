@@ -61,6 +61,12 @@ constexpr int bulge_seg_words(int nw) { return 2 + 3 * nw; }
 constexpr int bulge_family_words(int nw) { return 4 + kBulgeSegs * bulge_seg_words(nw); }
 constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that never hits
 
+// Statistics counters updated once per workgroup are spread over kCountSlots
+// slots kSlotStride words apart (one 128-byte line each): a single hot address
+// serializes its atomics in one L2 channel.  The host sums the slots.
+constexpr int kCountSlots = 64;
+constexpr int kSlotStride = 16;
+
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.
 constexpr int kHitKeyShift = 40;
 constexpr uint64_t kMaxGenomePositions = 1ull << (kHitKeyShift - 1);

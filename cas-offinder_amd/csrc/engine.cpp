@@ -41,7 +41,7 @@ void key_tables(const std::vector<std::shared_ptr<Query>> &qs, std::vector<uint3
 int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
                  const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
     double t_find = 0, t_cmp = 0;
-    uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0, launches = 0;
+    uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0, launches = 0, nloci = 0, npos = 0;
     std::vector<uint32_t> kw;
     std::vector<size_t> off;
     key_tables(qs, kw, off, true);
@@ -54,28 +54,46 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
         const int stride = q.single_stride;
         for (uint64_t lo = 0; lo < limit; lo += slab_positions) {
             const uint64_t hi = std::min(limit, lo + slab_positions);
-            unsigned long long found[2] = {0, 0};
-            for (;;) {
-                HIP_TRY(hipMemsetAsync(L.d_count.p + 1, 0, 2 * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
-                HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-                HIP_TRY(launch_finder(dg, q.pat, lo, hi, L.d_loci.p, L.d_flags.p, L.d_count.p + 1, L.d_loci.n, L.stream),
-                        "finder launch");
-                HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
-                HIP_TRY(hipMemcpyAsync(found, L.d_count.p + 1, sizeof found, hipMemcpyDeviceToHost, L.stream),
-                        "hipMemcpyAsync");
-                HIP_TRY(hipStreamSynchronize(L.stream), "finder");
-                t_find += ms_between(L.ev[0], L.ev[1]);
-                launches++;
-                if (found[0] <= L.d_loci.n) break;
-                const uint64_t want = found[0] + found[0] / 8 + 1024;
-                HIP_TRY(L.d_loci.reserve(want), "hipMalloc(candidates)");
-                HIP_TRY(L.d_flags.reserve(want), "hipMalloc(candidates)");
-            }
+            // finder pass 1: entries per tile, scanned into tile offsets
+            const uint64_t tiles = finder_tiles(lo, hi);
+            HIP_TRY(L.d_tile_cnt.reserve(tiles), "hipMalloc(finder tiles)");
+            HIP_TRY(L.d_tile_incl.reserve(tiles), "hipMalloc(finder tiles)");
+            size_t scan_bytes = 0;
+            HIP_TRY(scan_tile_counts(nullptr, &scan_bytes, L.d_tile_cnt.p, L.d_tile_incl.p, tiles, L.stream), "scan size");
+            HIP_TRY(L.d_scan_tmp.reserve(scan_bytes), "hipMalloc(scan)");
+            unsigned long long *slots = L.d_count.p + kCandBase;
+            HIP_TRY(hipMemsetAsync(slots, 0, kCandWords * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
+            HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
+            HIP_TRY(launch_finder_count(dg, q.pat, lo, hi, L.d_tile_cnt.p, slots, L.stream), "finder launch");
+            HIP_TRY(scan_tile_counts(L.d_scan_tmp.p, &scan_bytes, L.d_tile_cnt.p, L.d_tile_incl.p, tiles, L.stream),
+                    "finder scan");
+            HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
+            uint32_t entries = 0;
+            std::vector<unsigned long long> sl(kCandWords);
+            HIP_TRY(hipMemcpyAsync(&entries, L.d_tile_incl.p + tiles - 1, sizeof entries, hipMemcpyDeviceToHost,
+                                   L.stream),
+                    "hipMemcpyAsync");
+            HIP_TRY(hipMemcpyAsync(sl.data(), slots, kCandWords * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                   L.stream),
+                    "hipMemcpyAsync");
+            HIP_TRY(hipStreamSynchronize(L.stream), "finder");
+            unsigned long long found[2] = {entries, 0};
+            for (int k = 0; k < kCountSlots; k++) found[1] += sl[(size_t)k * kSlotStride];
+            // finder pass 2: entries at their offsets, in position order
+            HIP_TRY(L.d_loci.reserve(found[0]), "hipMalloc(candidates)");
+            HIP_TRY(L.d_flags.reserve(found[0]), "hipMalloc(candidates)");
+            HIP_TRY(hipEventRecord(L.ev[4], L.stream), "hipEventRecord");
+            HIP_TRY(launch_finder_write(dg, q.pat, lo, hi, L.d_tile_incl.p, L.d_loci.p, L.d_flags.p, L.d_loci.n,
+                                        L.stream),
+                    "finder launch");
+            HIP_TRY(hipEventRecord(L.ev[5], L.stream), "hipEventRecord");
+            launches += 3;
             ncand += found[1];
+            nloci += found[0];
+            npos += hi - lo;
             npairs += found[1] * q.keys.size();
             nbase += found[1] * q.care_sum;
             nmodel += found[1] * q.keys.size() * (uint64_t)q.nw * (q.gen ? 6 : 3);
-            if (found[0] == 0) continue;
             HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
             for (size_t k = 0; k < q.keys.size(); k++) {
                 HIP_TRY(launch_comparer(dg, q.P, q.nw, q.gen, (uint32_t)q.cbase, L.d_keys.p + off[qi] + k * (size_t)stride,
@@ -86,6 +104,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
             }
             HIP_TRY(hipEventRecord(L.ev[3], L.stream), "hipEventRecord");
             HIP_TRY(hipStreamSynchronize(L.stream), "comparer");
+            t_find += ms_between(L.ev[0], L.ev[1]) + ms_between(L.ev[4], L.ev[5]);
             t_cmp += ms_between(L.ev[2], L.ev[3]);
         }
     }
@@ -101,6 +120,8 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
     L.stats.n_launches = launches;
+    L.stats.n_loci = nloci;
+    L.stats.n_finder_positions = npos;
     return COF_OK;
 }
 
@@ -171,28 +192,33 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
-    // counters: [0] hits, [2 + launch] candidates (run_search sized them for one launch per query)
-    if (2 + nlaunch > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
+    // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
+    if (kCandBase + nlaunch * kCandWords > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
+    auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
     for (size_t g = 0; g < ktab.size(); g++) {
         HIP_TRY(launch_fused(dg, krep[g]->pat, ktab[g].nw, ktab[g].gen, (uint32_t)ktab[g].cbase, L.d_keys.p + koff[g],
-                             ktab[g].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + g, L.d_hkey.n,
+                             ktab[g].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(g), L.d_hkey.n,
                              L.stream),
                 "fused launch");
     }
     for (size_t g = 0; g < btab.size(); g++) {
         const BulgeTable &t = btab[g];
         HIP_TRY(launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[g], t.units,
-                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, L.d_count.p + 2 + ktab.size() + g, L.d_hkey.n,
+                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(ktab.size() + g), L.d_hkey.n,
                              L.stream),
                 "bulge launch");
     }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
-    std::vector<unsigned long long> cnt(2 + nlaunch);
-    HIP_TRY(hipMemcpyAsync(cnt.data(), L.d_count.p, cnt.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
+    HIP_TRY(hipMemcpyAsync(raw.data(), L.d_count.p, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            L.stream),
             "hipMemcpyAsync(count)");
     HIP_TRY(hipStreamSynchronize(L.stream), "fused");
+    std::vector<unsigned long long> cnt(2 + nlaunch, 0);  // [0] hits, [2 + launch] candidates
+    cnt[0] = raw[0];
+    for (size_t i = 0; i < nlaunch; i++)
+        for (int sl = 0; sl < kCountSlots; sl++) cnt[2 + i] += raw[kCandBase + i * kCandWords + (size_t)sl * kSlotStride];
     L.stats.t_fused_ms = ms_between(L.ev[0], L.ev[1]);
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
@@ -220,6 +246,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
     L.stats.n_launches = nlaunch;
+    L.stats.n_loci = 0;
+    L.stats.n_finder_positions = 0;
     nhits = cnt[0];
     return COF_OK;
 }
@@ -246,6 +274,9 @@ void Lane::release() {
     d_count.release();
     d_loci.release();
     d_flags.release();
+    d_tile_cnt.release();
+    d_tile_incl.release();
+    d_scan_tmp.release();
     d_keys.release();
     for (auto &x : ev)
         if (x) (void)hipEventDestroy(x), x = nullptr;
@@ -269,7 +300,7 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     }
     gbase[qs.size()] = (uint32_t)nkeys;
     const auto t0 = std::chrono::steady_clock::now();
-    HIP_TRY(L.d_count.reserve(3 + qs.size()), "hipMalloc(counters)");
+    HIP_TRY(L.d_count.reserve(kCandBase + qs.size() * kCandWords), "hipMalloc(counters)");
     if (!L.d_hkey.p) {
         HIP_TRY(L.d_hkey.reserve(1 << 20), "hipMalloc(hits)");
         HIP_TRY(L.d_hmm.reserve(1 << 20), "hipMalloc(hits)");

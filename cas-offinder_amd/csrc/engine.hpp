@@ -49,8 +49,14 @@ struct cof_result {
 
 namespace cof {
 
-hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
-                         uint32_t *loci, uint8_t *flags, unsigned long long *count, uint64_t cap, hipStream_t s);
+uint64_t finder_tiles(uint64_t slab_lo, uint64_t slab_hi);
+hipError_t launch_finder_count(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
+                               uint32_t *tile_cnt, unsigned long long *cand_slots, hipStream_t s);
+hipError_t scan_tile_counts(void *tmp, size_t *tmp_bytes, const uint32_t *cnt, uint32_t *incl, uint64_t n,
+                            hipStream_t s);
+hipError_t launch_finder_write(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
+                               const uint32_t *tile_incl, uint32_t *loci, uint8_t *flags, uint64_t cap,
+                               hipStream_t s);
 hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
                            uint32_t gbase, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
                            uint64_t *hkey, uint16_t *hmm, unsigned long long *count, uint64_t cap, hipStream_t s);
@@ -110,11 +116,15 @@ struct Lane {
     DevBuf<uint8_t> d_sort_tmp;
     HostBuf<uint64_t> h_hkey;
     HostBuf<uint16_t> h_hmm;
-    DevBuf<unsigned long long> d_count;  // [0] hits, [1..2] plain finder counters, [2+q] batched candidates
+    // [0] hits; from kCandBase, per batched launch (or for the plain finder),
+    // kCountSlots candidate-count slots kSlotStride words apart
+    DevBuf<unsigned long long> d_count;
     DevBuf<uint32_t> d_loci;
     DevBuf<uint8_t> d_flags;
+    DevBuf<uint32_t> d_tile_cnt, d_tile_incl;  // plain finder: entries per tile and their scan
+    DevBuf<uint8_t> d_scan_tmp;
     DevBuf<uint32_t> d_keys;
-    hipEvent_t ev[4] = {};
+    hipEvent_t ev[6] = {};
     cof_stats stats{};
     hipError_t init();
     void release();
@@ -122,6 +132,9 @@ struct Lane {
 
 // Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a
 // COF_* status with the message in err.
+constexpr size_t kCandBase = kSlotStride;
+constexpr size_t kCandWords = (size_t)kCountSlots * kSlotStride;  // per launch
+
 int run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
                int mode, cof_result **out, std::string &err);
 

@@ -1,19 +1,24 @@
 // gfx950 kernels of the search path.
 //
 // Genome planes (lo, hi, ot) are 32-position words; a 256-thread workgroup
-// owns one tile of 256 words (8192 window starts) and stages the tile plus
-// kHaloL / kHaloR words of halo in LDS.  Each thread tests the PAM of the 32
-// starts of its word on both strands with bitwise class masks
-// (finder_masks), so one lane does 32 positions of cas-offinder.cl:26-63 at
-// once.  Windows holding a ';' (poison list) or running past the genome end
-// are cleared (cas-offinder.cl:64-67).
+// owns one tile of kTileWords = 1024 words (32768 window starts, four
+// consecutive words per thread) and stages the tile plus kHaloL / kHaloR
+// words of halo in LDS.  Each thread tests the PAM of the 32 starts of each of
+// its words on both strands with bitwise class masks (finder_masks), so one
+// lane does 32 positions of cas-offinder.cl:26-63 at once.  Windows holding a
+// ';' (poison list) or running past the genome end are cleared
+// (cas-offinder.cl:64-67).
 //
-//   finder_kernel    plain path: compacts (locus, flag) like cas-offinder.cl:68-70
+//   finder_count_kernel, finder_write_kernel
+//                    plain path: (locus, flag) entries like cas-offinder.cl:68-70,
+//                    at offsets from a scan of per-tile counts
 //   comparer_kernel  plain path: one compare key per launch over the
 //                    candidate list (cas-offinder.cl:73-155)
 //   fused_kernel     batched path: finder + compaction into LDS + every
-//                    compare key of the query per candidate, no candidate
-//                    list in HBM
+//                    compare key of a pattern group per candidate, no
+//                    candidate list in HBM
+//   bulge_kernel     batched path for queries with DNA/RNA bulges: the
+//                    variants are expanded on the device from the guides
 //
 // Mismatch counting works on NW 32-bit words of the window planes taken at
 // key position cbase.  A key whose comparable bases are all A/C/G/T is
@@ -27,6 +32,7 @@
 #include <hip/hip_runtime.h>
 
 #include <rocprim/device/device_radix_sort.hpp>
+#include <rocprim/device/device_scan.hpp>
 
 #include "devtypes.hpp"
 
@@ -415,26 +421,29 @@ __device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const ui
 struct FinderArgs {
     DevGenome g;
     PatternDesc pd;
-    uint64_t slab_lo, slab_hi;  // window starts [slab_lo, slab_hi), slab_lo tile aligned
-    uint32_t *loci;             // locus - slab_lo
-    uint8_t *flags;             // 0 both strands, 1 forward only, 2 reverse only
-    unsigned long long *count;  // [0] loci written, [1] (locus, strand) candidates
+    uint64_t slab_lo, slab_hi;   // window starts [slab_lo, slab_hi), slab_lo tile aligned
+    uint32_t *tile_cnt;          // count pass: (locus, flag) entries of each tile
+    const uint32_t *tile_incl;   // write pass: inclusive scan of tile_cnt
+    unsigned long long *cand_slots;  // (locus, strand) candidates, kCountSlots slots
+    uint32_t *loci;              // locus - slab_lo
+    uint8_t *flags;              // 0 both strands, 1 forward only, 2 reverse only
     uint64_t cap;
 };
 
 constexpr int kListCap = 4096;  // LDS candidate entries per pass
 
-__global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
-    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
-    __shared__ uint32_t s_ent[kListCap];
-    __shared__ uint32_t s_wsum[4], s_wsum2[4];
-    __shared__ unsigned long long s_base;
+// PAM masks of the starts of this workgroup's tile of the slab, with
+// poison and slab range applied; mine = starts with a PAM on either strand.
+__device__ __forceinline__ uint64_t slab_tile_masks(const FinderArgs &a, uint32_t *s_lo, uint32_t *s_hi,
+                                                    uint32_t *s_ot, uint32_t (&fwd)[kWordsPerThread],
+                                                    uint32_t (&rev)[kWordsPerThread], uint32_t &mine,
+                                                    uint32_t &nmine) {
     const int t = threadIdx.x;
     const int64_t w0 = (int64_t)(a.slab_lo >> 5) + (int64_t)blockIdx.x * kTileWords;
     stage_tile(a.g, w0, s_lo, s_hi, s_ot);
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
-    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
+    uint32_t keep[kWordsPerThread];
     uint64_t s0[kWordsPerThread];
     finder_masks(a.pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
 #pragma unroll
@@ -443,7 +452,7 @@ __global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
         keep[j] = range_mask(s0[j], a.slab_lo, a.slab_hi);
     }
     apply_poison(a.g, a.pd.P, t0 / kTilePos, t1, s0, keep);
-    uint32_t mine = 0, nmine = 0;
+    mine = nmine = 0;
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
         fwd[j] &= keep[j];
@@ -451,13 +460,38 @@ __global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
         mine += __popc(fwd[j] | rev[j]);
         nmine += __popc(fwd[j]) + __popc(rev[j]);
     }
+    return t0;
+}
+
+// Plain-path finder, pass 1: entries per tile (no atomics on a shared
+// address: the tile offsets come from a scan of these counts).
+__global__ __launch_bounds__(kThreads) void finder_count_kernel(FinderArgs a) {
+    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint32_t s_wsum[4], s_wsum2[4];
+    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], mine, nmine;
+    (void)slab_tile_masks(a, s_lo, s_hi, s_ot, fwd, rev, mine, nmine);
     uint32_t total, nstrand;
-    const uint32_t first = block_scan(mine, s_wsum, &total);
+    (void)block_scan(mine, s_wsum, &total);
     (void)block_scan(nmine, s_wsum2, &nstrand);
-    if (t == 0) {
-        s_base = total ? atomicAdd(a.count, (unsigned long long)total) : 0ull;
-        if (nstrand) atomicAdd(a.count + 1, (unsigned long long)nstrand);
+    if (threadIdx.x == 0) {
+        a.tile_cnt[blockIdx.x] = total;
+        if (nstrand) atomicAdd(a.cand_slots + (blockIdx.x % kCountSlots) * kSlotStride, (unsigned long long)nstrand);
     }
+}
+
+// Pass 2: the same masks, entries written at the tile's scanned offset in
+// position order (cas-offinder.cl:68-70 compacts the same (locus, flag)
+// pairs in atomic order).
+__global__ __launch_bounds__(kThreads) void finder_write_kernel(FinderArgs a) {
+    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint32_t s_ent[kListCap];
+    __shared__ uint32_t s_wsum[4];
+    const int t = threadIdx.x;
+    uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], mine, nmine;
+    const uint64_t t0 = slab_tile_masks(a, s_lo, s_hi, s_ot, fwd, rev, mine, nmine);
+    uint32_t total;
+    const uint32_t first = block_scan(mine, s_wsum, &total);
+    const uint64_t base = blockIdx.x ? a.tile_incl[blockIdx.x - 1] : 0u;
     // entries (tile offset << 2 | flag) staged in LDS, then written out coalesced
     for (uint32_t pass = 0; pass < total; pass += kListCap) {
         const uint32_t cnt = min(total - pass, (uint32_t)kListCap);
@@ -478,7 +512,7 @@ __global__ __launch_bounds__(kThreads) void finder_kernel(FinderArgs a) {
         __syncthreads();
         for (uint32_t i = t; i < cnt; i += kThreads) {
             const uint32_t e = s_ent[i];
-            const uint64_t o = s_base + pass + i;
+            const uint64_t o = base + pass + i;
             if (o < a.cap) {
                 a.loci[o] = (uint32_t)(t0 + (e >> 2) - a.slab_lo);
                 a.flags[o] = (uint8_t)(e & 3);
@@ -566,7 +600,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     }
     uint32_t n;
     const uint32_t first = block_scan(mine, s_wsum, &n);
-    if (t == 0 && n) atomicAdd(cand_count, (unsigned long long)n);
+    if (t == 0 && n) atomicAdd(cand_count + (blockIdx.x % kCountSlots) * kSlotStride, (unsigned long long)n);
 
     const int lane = t & 63, wave = t >> 6;
     auto rd = [=](int64_t wi, uint32_t &l, uint32_t &h, uint32_t &o) {
@@ -844,13 +878,46 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
 
 // ------------------------------------------------------------------ launchers
 
-hipError_t launch_finder(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
-                         uint32_t *loci, uint8_t *flags, unsigned long long *count, uint64_t cap,
-                         hipStream_t s) {
-    if (slab_hi <= slab_lo) return hipSuccess;
-    FinderArgs a{g, pd, slab_lo, slab_hi, loci, flags, count, cap};
-    const uint64_t tiles = (slab_hi - slab_lo + kTilePos - 1) / kTilePos;
-    hipLaunchKernelGGL(finder_kernel, dim3((uint32_t)tiles), dim3(kThreads), 0, s, a);
+static FinderArgs finder_args(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi) {
+    FinderArgs a{};
+    a.g = g;
+    a.pd = pd;
+    a.slab_lo = slab_lo;
+    a.slab_hi = slab_hi;
+    return a;
+}
+
+uint64_t finder_tiles(uint64_t slab_lo, uint64_t slab_hi) {
+    return slab_hi > slab_lo ? (slab_hi - slab_lo + kTilePos - 1) / kTilePos : 0;
+}
+
+hipError_t launch_finder_count(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
+                               uint32_t *tile_cnt, unsigned long long *cand_slots, hipStream_t s) {
+    const uint64_t tiles = finder_tiles(slab_lo, slab_hi);
+    if (!tiles) return hipSuccess;
+    FinderArgs a = finder_args(g, pd, slab_lo, slab_hi);
+    a.tile_cnt = tile_cnt;
+    a.cand_slots = cand_slots;
+    hipLaunchKernelGGL(finder_count_kernel, dim3((uint32_t)tiles), dim3(kThreads), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t scan_tile_counts(void *tmp, size_t *tmp_bytes, const uint32_t *cnt, uint32_t *incl, uint64_t n,
+                            hipStream_t s) {
+    return rocprim::inclusive_scan(tmp, *tmp_bytes, cnt, incl, (size_t)n, rocprim::plus<uint32_t>(), s);
+}
+
+hipError_t launch_finder_write(const DevGenome &g, const PatternDesc &pd, uint64_t slab_lo, uint64_t slab_hi,
+                               const uint32_t *tile_incl, uint32_t *loci, uint8_t *flags, uint64_t cap,
+                               hipStream_t s) {
+    const uint64_t tiles = finder_tiles(slab_lo, slab_hi);
+    if (!tiles) return hipSuccess;
+    FinderArgs a = finder_args(g, pd, slab_lo, slab_hi);
+    a.tile_incl = tile_incl;
+    a.loci = loci;
+    a.flags = flags;
+    a.cap = cap;
+    hipLaunchKernelGGL(finder_write_kernel, dim3((uint32_t)tiles), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 

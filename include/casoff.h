@@ -60,9 +60,13 @@ typedef struct {
     uint64_t n_launches;
     uint64_t genome_len;    /* positions scanned (global coordinate incl. separators) */
     uint64_t n_base_cmp;    /* sum over pairs of the key's comparable (non-N) positions */
-    uint64_t n_model_ops;   /* int32 VALU ops of the bit-sliced compare: 3 per (pair, 32-position
-                               word) for A/C/G/T keys, 6 for keys with IUPAC codes */
+    uint64_t n_model_ops;   /* int32 VALU ops of the comparer model: fused kernel 3 per (pair,
+                               32-position word) for A/C/G/T keys, 6 with IUPAC codes; bulge kernel
+                               per (candidate, guide, word) 2 per window alignment (5 with IUPAC
+                               codes) + 1 + 3 per bulge family */
     double t_collect_ms;    /* device sort of the hits + copy to host + decode */
+    uint64_t n_loci;        /* plain mode: (locus, flag) entries the finder wrote */
+    uint64_t n_finder_positions; /* plain mode: window starts the finder launches scanned */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference
