@@ -48,13 +48,50 @@ __device__ __forceinline__ uint32_t funnel(uint32_t hw, uint32_t lw, uint32_t sh
     return __builtin_amdgcn_alignbit(hw, lw, sh);  // bits [sh, sh+32) of hw:lw
 }
 
-__device__ __forceinline__ uint32_t sel_mask(uint32_t m, uint32_t c) { return ((m >> c) & 1u) ? ~0u : 0u; }
+// Class match of the planes for a 5-bit class set M (bit kClsT..kClsO: the
+// classes that MATCH) as one v_bitop3 over (lo, hi, ot), truth-table index
+// lo*4 + hi*2 + ot: T = 000, other = 001 (ot is set only where lo = hi = 0),
+// A = 010, C = 100, G = 110; the other indices never occur.
+template <uint32_t M>
+constexpr uint32_t match_tt() {
+    return ((M >> kClsT) & 1) << 0 | ((M >> kClsO) & 1) << 1 | ((M >> kClsA) & 1) << 2 |
+           ((M >> kClsC) & 1) << 4 | ((M >> kClsG) & 1) << 6;
+}
 
-// planes -> class-match bits for a 5-bit class set `m` (ot positions have lo = hi = 0)
-__device__ __forceinline__ uint32_t class_match(uint32_t m, uint32_t l, uint32_t h, uint32_t o) {
-    const uint32_t T = ~(l | h | o), C = l & ~h, A = h & ~l, G = l & h;
-    return (T & sel_mask(m, kClsT)) | (C & sel_mask(m, kClsC)) | (A & sel_mask(m, kClsA)) |
-           (G & sel_mask(m, kClsG)) | (o & sel_mask(m, kClsO));
+// acc[j] &= class match, for the class set M, of the planes shifted to
+// pattern position k (bit b of word j <-> start b of word i0 + j).
+template <uint32_t M, int NJ, int NL>
+__device__ __forceinline__ void and_match(uint32_t (&acc)[NJ], const uint32_t (&L)[NL], const uint32_t (&H)[NL],
+                                          const uint32_t (&O)[NL], uint32_t k) {
+    const uint32_t sh = k & 31;
+    const bool up = k >= 32;
+#pragma unroll
+    for (int j = 0; j < NJ; j++) {
+        const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
+        const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
+        const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
+        acc[j] &= __builtin_amdgcn_bitop3_b32(l, h, o, match_tt<M>());
+    }
+}
+
+// The class set of a pattern position is uniform, so a scalar switch picks
+// the truth table.
+template <int NJ, int NL>
+__device__ __forceinline__ void and_match_any(uint32_t m, uint32_t (&acc)[NJ], const uint32_t (&L)[NL],
+                                              const uint32_t (&H)[NL], const uint32_t (&O)[NL], uint32_t k) {
+#define COF_MATCH_CASE(x) \
+    case x: and_match<x>(acc, L, H, O, k); break;
+    switch (m & 31u) {
+        COF_MATCH_CASE(0) COF_MATCH_CASE(1) COF_MATCH_CASE(2) COF_MATCH_CASE(3)
+        COF_MATCH_CASE(4) COF_MATCH_CASE(5) COF_MATCH_CASE(6) COF_MATCH_CASE(7)
+        COF_MATCH_CASE(8) COF_MATCH_CASE(9) COF_MATCH_CASE(10) COF_MATCH_CASE(11)
+        COF_MATCH_CASE(12) COF_MATCH_CASE(13) COF_MATCH_CASE(14) COF_MATCH_CASE(15)
+        COF_MATCH_CASE(16) COF_MATCH_CASE(17) COF_MATCH_CASE(18) COF_MATCH_CASE(19)
+        COF_MATCH_CASE(20) COF_MATCH_CASE(21) COF_MATCH_CASE(22) COF_MATCH_CASE(23)
+        COF_MATCH_CASE(24) COF_MATCH_CASE(25) COF_MATCH_CASE(26) COF_MATCH_CASE(27)
+        COF_MATCH_CASE(28) COF_MATCH_CASE(29) COF_MATCH_CASE(30) COF_MATCH_CASE(31)
+    }
+#undef COF_MATCH_CASE
 }
 
 // PAM test of the kWordsPerThread consecutive LDS words starting at i0 (the
@@ -73,28 +110,8 @@ __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32
     }
 #pragma unroll
     for (int j = 0; j < NJ; j++) fwd[j] = rev[j] = ~0u;
-    for (uint32_t i = 0; i < pd.nf; i++) {
-        const uint32_t k = pd.fk[i], sh = k & 31, m = pd.fm[i];
-        const bool up = k >= 32;
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
-            const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
-            const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
-            fwd[j] &= class_match(m, l, h, o);
-        }
-    }
-    for (uint32_t i = 0; i < pd.nr; i++) {
-        const uint32_t k = pd.rk[i], sh = k & 31, m = pd.rm[i];
-        const bool up = k >= 32;
-#pragma unroll
-        for (int j = 0; j < NJ; j++) {
-            const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
-            const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
-            const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
-            rev[j] &= class_match(m, l, h, o);
-        }
-    }
+    for (uint32_t i = 0; i < pd.nf; i++) and_match_any(pd.fm[i], fwd, L, H, O, pd.fk[i]);
+    for (uint32_t i = 0; i < pd.nr; i++) and_match_any(pd.rm[i], rev, L, H, O, pd.rk[i]);
 }
 
 // Bits of the starts [s0, s0+32) inside [lo, hi).
@@ -128,15 +145,42 @@ __device__ __forceinline__ void apply_poison(const DevGenome &g, uint32_t P, uin
     }
 }
 
+// The planes of one tile with its halo (kLdsWords words each), spread over
+// the workgroup: thread t holds LDS words t, t + kThreads, ...
+constexpr int kStageIters = (kLdsWords + kThreads - 1) / kThreads;
+struct TileRegs {
+    uint32_t lo[kStageIters], hi[kStageIters], ot[kStageIters];
+};
+
+__device__ __forceinline__ void load_tile(const DevGenome &g, int64_t w0, TileRegs &r) {
+#pragma unroll
+    for (int k = 0; k < kStageIters; k++) {
+        const int i = (int)threadIdx.x + k * kThreads;
+        const int64_t w = w0 - kHaloL + i;
+        const bool in = i < kLdsWords && w >= 0 && (uint64_t)w < g.nwords;
+        r.lo[k] = in ? g.lo[w] : 0u;
+        r.hi[k] = in ? g.hi[w] : 0u;
+        r.ot[k] = in ? g.ot[w] : ~0u;
+    }
+}
+
+__device__ __forceinline__ void store_tile(const TileRegs &r, uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot) {
+#pragma unroll
+    for (int k = 0; k < kStageIters; k++) {
+        const int i = (int)threadIdx.x + k * kThreads;
+        if (i < kLdsWords) {
+            s_lo[i] = r.lo[k];
+            s_hi[i] = r.hi[k];
+            s_ot[i] = r.ot[k];
+        }
+    }
+}
+
 __device__ __forceinline__ void stage_tile(const DevGenome &g, int64_t w0, uint32_t *s_lo, uint32_t *s_hi,
                                            uint32_t *s_ot) {
-    for (int i = threadIdx.x; i < kLdsWords; i += blockDim.x) {
-        const int64_t w = w0 - kHaloL + i;
-        const bool in = w >= 0 && (uint64_t)w < g.nwords;
-        s_lo[i] = in ? g.lo[w] : 0u;
-        s_hi[i] = in ? g.hi[w] : 0u;
-        s_ot[i] = in ? g.ot[w] : ~0u;
-    }
+    TileRegs r;
+    load_tile(g, w0, r);
+    store_tile(r, s_lo, s_hi, s_ot);
 }
 
 // exclusive scan of `v` over the workgroup (four wave64s); *total gets the sum
