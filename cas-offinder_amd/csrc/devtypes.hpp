@@ -19,14 +19,15 @@ constexpr int kHaloR = 6;         // words staged after the tile
 constexpr int kLdsWords = kTileWords + kHaloL + kHaloR;
 
 // Finder pattern: the positions of the padded pattern (forward, and its
-// reverse complement in forward text orientation) that can mismatch, each with
-// the 5-bit set of classes that MATCH there.
+// reverse complement in forward text orientation) that can mismatch, each
+// packed as position | (5-bit set of classes that MATCH there) << 8.  Entries
+// are 32-bit so that kernels read them with scalar loads.
 struct PatternDesc {
     uint32_t P;
     uint32_t nf, nr;
     uint32_t pad;
-    uint8_t fk[kMaxPattern], fm[kMaxPattern];
-    uint8_t rk[kMaxPattern], rm[kMaxPattern];
+    uint32_t f[kMaxPattern];
+    uint32_t r[kMaxPattern];
 };
 
 // Compare-key tables (bit b of a word <-> key position cbase + 32*word + b).
@@ -46,8 +47,8 @@ constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 // at kMaxShifts window alignments at most (its own plus one per bulge family).
 // Records, grouped by care class like concrete key tables ([care[NW], nguides,
 // 0, 0] headers), are [guide planes][variant block offset][bias per shift],
-// padded to 4 dwords; planes are glo[NW], ghi[NW] (concrete) or the five class
-// masks [class * NW + word] (general).  A variant block is [gbase, 0, 0, 0]
+// padded to 4 dwords; planes are glo[NW], ghi[NW], care[NW] (concrete) or the
+// five class masks [class * NW + word] (general).  A variant block is [gbase, 0, 0, 0]
 // followed by one family block per shift: [first split (bit coordinate,
 // int32), split count, entry offset, segment count] and kBulgeSegs segment
 // records [first split index, split count, below[NW], inner[NW], above[NW]]
@@ -55,7 +56,7 @@ constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 // query-local key index] of every split.
 constexpr int kMaxShifts = 5;
 constexpr int kBulgeSegs = 4;
-constexpr int bulge_planes(int nw, bool gen) { return gen ? 5 * nw : 2 * nw; }
+constexpr int bulge_planes(int nw, bool gen) { return gen ? 5 * nw : 3 * nw; }
 constexpr int bulge_stride(int nw, bool gen, int ns) { return (bulge_planes(nw, gen) + 1 + ns + 3) & ~3; }
 constexpr int bulge_seg_words(int nw) { return 2 + 3 * nw; }
 constexpr int bulge_family_words(int nw) { return 4 + kBulgeSegs * bulge_seg_words(nw); }

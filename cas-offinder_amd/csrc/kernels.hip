@@ -110,8 +110,8 @@ __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32
     }
 #pragma unroll
     for (int j = 0; j < NJ; j++) fwd[j] = rev[j] = ~0u;
-    for (uint32_t i = 0; i < pd.nf; i++) and_match_any(pd.fm[i], fwd, L, H, O, pd.fk[i]);
-    for (uint32_t i = 0; i < pd.nr; i++) and_match_any(pd.rm[i], rev, L, H, O, pd.rk[i]);
+    for (uint32_t i = 0; i < pd.nf; i++) and_match_any(pd.f[i] >> 8, fwd, L, H, O, pd.f[i] & 0xFF);
+    for (uint32_t i = 0; i < pd.nr; i++) and_match_any(pd.r[i] >> 8, rev, L, H, O, pd.r[i] & 0xFF);
 }
 
 // Bits of the starts [s0, s0+32) inside [lo, hi).
@@ -229,7 +229,7 @@ __device__ __forceinline__ void emit_hit(const HitSink &o, uint32_t key, uint32_
 // pass instead of one per hit (same-address atomics serialize in one L2
 // channel).  An entry packs global key index (bits 23..46), mismatches
 // (16..22), strand (15) and the start's offset in the tile (0..14).
-constexpr int kStageCap = 768;
+constexpr int kStageCap = 512;
 struct HitStage {
     uint64_t ent[kStageCap];
     uint32_t n;  // entries appended this pass; past kStageCap they went to HBM directly
@@ -614,7 +614,10 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 // Entries are tile offset << 1 | strand (forward entries of a word first).
 // Hits go through the workgroup's LDS stage st, flushed to (hkey, hmm,
 // count, cap) after every pass.
-template <int W, typename Cands>
+// With ALL, cands runs on every lane of a wave whose chunk holds a candidate
+// (lanes past the list end get valid = false), so it may use wave-wide
+// ballots and cross-lane queues.
+template <int W, bool ALL = false, typename Cands>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 unsigned long long *cand_count, uint32_t *s_lo, uint32_t *s_hi,
                                                 uint32_t *s_ot, uint16_t *s_list, uint32_t *s_wsum, HitStage *st,
@@ -670,14 +673,14 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
         __syncthreads();
         for (uint32_t base = (uint32_t)wave * 64 * W; base < cnt; base += kThreads * W) {
             const uint32_t i0 = base + lane;
-            if (i0 < cnt) {
+            if (ALL || i0 < cnt) {
                 uint32_t e[W];
                 bool valid[W];
 #pragma unroll
                 for (int w = 0; w < W; w++) {
                     const uint32_t iw = i0 + 64 * w;
                     valid[w] = iw < cnt;
-                    e[w] = s_list[valid[w] ? iw : i0];  // an empty slot repeats candidate i0
+                    e[w] = s_list[valid[w] ? iw : min(i0, cnt - 1)];  // an empty slot repeats a candidate
                 }
                 cands(t0, rd, e, valid);
             }
@@ -814,39 +817,122 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
     }
 }
 
+// Window planes of candidate entry e (tile offset << 1 | strand) at the NS
+// alignments of the bulge table.
+template <int NW, int NS, typename Rd>
+__device__ __forceinline__ void bulge_windows(const BulgeArgs &a, Rd rd, uint32_t e, uint32_t (&l)[NS][NW],
+                                              uint32_t (&h)[NS][NW], uint32_t (&o)[NS][NW]) {
+    const uint32_t ls = e >> 1, strand = e & 1;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int64_t bit = strand == 0 ? (int64_t)ls + a.cb[u] + 32 * kHaloL
+                                        : (int64_t)ls + a.pd.P - 32 * NW - (int64_t)a.cb[u] + 32 * kHaloL;
+        window_planes<NW>(rd, bit, strand, l[u], h[u], o[u]);
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void class_planes(const uint32_t (&l)[NW], const uint32_t (&h)[NW], const uint32_t (&o)[NW],
+                                             uint32_t (&cls)[5][NW]) {
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        cls[kClsT][w] = ~(l[w] | h[w] | o[w]);
+        cls[kClsC][w] = l[w] & ~h[w];
+        cls[kClsA][w] = h[w] & ~l[w];
+        cls[kClsG][w] = l[w] & h[w];
+        cls[kClsO][w] = o[w];
+    }
+}
+
+// Mismatch mask of a general (IUPAC) guide record against the class planes.
+template <int NW>
+__device__ __forceinline__ uint32_t general_mask(kptr_t kp, const uint32_t (&cls)[5][NW], int w) {
+    uint32_t x = cls[kClsO][w] & kp[kClsO * NW + w];
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsG][w], kp[kClsG * NW + w], x, kAndOr);
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsA][w], kp[kClsA * NW + w], x, kAndOr);
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsC][w], kp[kClsC * NW + w], x, kAndOr);
+    return __builtin_amdgcn_bitop3_b32(cls[kClsT][w], kp[kClsT * NW + w], x, kAndOr);
+}
+
+// Each wave queues the (candidate, guide) pairs whose family filter passed
+// instead of branching into the exact count for one or two lanes at a time;
+// a full queue (or the end of the candidate chunk) is drained one pair per
+// lane.  An item is the guide record offset and the candidate entry | the
+// mask of passing families << 16.
+constexpr int kBulgeQueue = 64;
+
 template <int NW, bool GEN, int NS>
 __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
     __shared__ HitStage s_stage;
-    constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS);
-    const uint32_t P = a.pd.P;
-    tile_candidates<1>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey, a.hmm,
-                       a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
-        const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
-        uint32_t l[NS][NW], h[NS][NW], o[NS][NW];
+    __shared__ uint32_t s_qrec[4][kBulgeQueue], s_qinfo[4][kBulgeQueue];
+    constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS), FB = bulge_family_words(NW);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave];
+    const kptr_t tab = (kptr_t)a.table;
+    tile_candidates<1, true>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey,
+                             a.hmm, a.count, a.cap,
+                             [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
+        // exact counts of the queued pairs, one per lane (records read per lane)
+        auto drain = [&](uint32_t qn) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+            __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+            if (lane < qn) {
+                const uint32_t info = qinfo[lane];
+                const kptr_t kp = tab + qrec[lane];
+                const uint32_t ce = info & 0xFFFFu, fam = info >> 16;
+                uint32_t l[NS][NW], h[NS][NW], o[NS][NW], M[NS][NW];
+                bulge_windows<NW, NS>(a, rd, ce, l, h, o);
 #pragma unroll
-        for (int u = 0; u < NS; u++) {
-            const int64_t bit = strand == 0 ? (int64_t)ls + a.cb[u] + 32 * kHaloL
-                                            : (int64_t)ls + P - 32 * NW - (int64_t)a.cb[u] + 32 * kHaloL;
-            window_planes<NW>(rd, bit, strand, l[u], h[u], o[u]);
-        }
-        TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, &s_stage, (strand << 15) | ls};
+                for (int u = 0; u < NS; u++) {
+                    if (GEN) {
+                        uint32_t cls[5][NW];
+                        class_planes<NW>(l[u], h[u], o[u], cls);
+#pragma unroll
+                        for (int w = 0; w < NW; w++) M[u][w] = general_mask<NW>(kp, cls, w);
+                    } else {
+#pragma unroll
+                        for (int w = 0; w < NW; w++) {
+                            const uint32_t care = kp[2 * NW + w];
+                            const uint32_t x = __builtin_amdgcn_bitop3_b32(h[u][w] & care, kp[NW + w], o[u][w] & care,
+                                                                           kXorOr);
+                            M[u][w] = __builtin_amdgcn_bitop3_b32(l[u][w] & care, kp[w], x, kXorOr);
+                        }
+                    }
+                }
+                const kptr_t vb = tab + kp[KP];
+                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]}, &s_stage,
+                                    ((ce & 1) << 15) | (ce >> 1)};
+#pragma unroll
+                for (int u = 0; u < NS; u++) {
+                    if ((fam >> u) & 1) {
+                        const bool lft0 = u == 0 || a.rev;
+                        uint32_t left[NW], right[NW];
+#pragma unroll
+                        for (int w = 0; w < NW; w++) {
+                            left[w] = lft0 ? M[0][w] : M[u][w];
+                            right[w] = lft0 ? M[u][w] : M[0][w];
+                        }
+                        bulge_family<NW>(tab, vb + 4 + FB * u, left, right, a.gap[u], kp[KP + 1 + u], sink);
+                    }
+                }
+            }
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+            __builtin_amdgcn_wave_barrier();
+        };
+
+        uint32_t l[NS][NW], h[NS][NW], o[NS][NW];
+        bulge_windows<NW, NS>(a, rd, e[0], l, h, o);
         uint32_t cls[GEN ? NS : 1][5][NW];
         if (GEN) {
 #pragma unroll
-            for (int u = 0; u < NS; u++)
-#pragma unroll
-                for (int w = 0; w < NW; w++) {
-                    cls[u][kClsT][w] = ~(l[u][w] | h[u][w] | o[u][w]);
-                    cls[u][kClsC][w] = l[u][w] & ~h[u][w];
-                    cls[u][kClsA][w] = h[u][w] & ~l[u][w];
-                    cls[u][kClsG][w] = l[u][w] & h[u][w];
-                    cls[u][kClsO][w] = o[u][w];
-                }
+            for (int u = 0; u < NS; u++) class_planes<NW>(l[u], h[u], o[u], cls[u]);
         }
-        kptr_t kp = (kptr_t)a.table;
+        uint32_t qn = 0;  // wave-uniform queue length
+        kptr_t kp = tab;
         for (uint32_t c = 0; c < a.nunits; c++) {
             uint32_t lc[NS][NW], hc[NS][NW], oc[NS][NW];
             if (!GEN) {
@@ -870,11 +956,7 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
 #pragma unroll
                     for (int w = 0; w < NW; w++) {
                         if (GEN) {
-                            uint32_t x = cls[u][kClsO][w] & kp[kClsO * NW + w];
-                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsG][w], kp[kClsG * NW + w], x, kAndOr);
-                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsA][w], kp[kClsA * NW + w], x, kAndOr);
-                            x = __builtin_amdgcn_bitop3_b32(cls[u][kClsC][w], kp[kClsC * NW + w], x, kAndOr);
-                            M[u][w] = __builtin_amdgcn_bitop3_b32(cls[u][kClsT][w], kp[kClsT * NW + w], x, kAndOr);
+                            M[u][w] = general_mask<NW>(kp, cls[u], w);
                         } else {
                             const uint32_t x = __builtin_amdgcn_bitop3_b32(hc[u][w], kp[NW + w], oc[u][w], kXorOr);
                             M[u][w] = __builtin_amdgcn_bitop3_b32(lc[u][w], kp[w], x, kXorOr);
@@ -896,27 +978,28 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                     f[u] = (int32_t)s;
                     acc |= f[u];
                 }
-                if (acc < 0) {
-                    const kptr_t vb = (kptr_t)a.table + kp[KP];
-                    sink.g.gbase = vb[0];
-                    constexpr int FB = bulge_family_words(NW);
-#pragma unroll
-                    for (int u = 0; u < NS; u++) {
-                        if (f[u] < 0) {
-                            const bool lft0 = u == 0 || a.rev;
-                            uint32_t left[NW], right[NW];
-#pragma unroll
-                            for (int w = 0; w < NW; w++) {
-                                left[w] = lft0 ? M[0][w] : M[u][w];
-                                right[w] = lft0 ? M[u][w] : M[0][w];
-                            }
-                            bulge_family<NW>((kptr_t)a.table, vb + 4 + FB * u, left, right, a.gap[u],
-                                             kp[KP + 1 + u], sink);
-                        }
+                const bool pass = valid[0] && acc < 0;
+                const uint64_t m = __ballot(pass);
+                if (m) {
+                    const uint32_t np = (uint32_t)__popcll(m);
+                    if (qn + np > (uint32_t)kBulgeQueue) {
+                        drain(qn);
+                        qn = 0;
                     }
+                    if (pass) {
+                        uint32_t fam = 0;
+#pragma unroll
+                        for (int u = 0; u < NS; u++) fam |= (f[u] < 0 ? 1u : 0u) << u;
+                        const uint32_t slot =
+                            qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        qrec[slot] = (uint32_t)(kp - tab);
+                        qinfo[slot] = e[0] | (fam << 16);
+                    }
+                    qn += np;
                 }
             }
         }
+        if (qn) drain(qn);
     });
 }
 

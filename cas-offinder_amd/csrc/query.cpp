@@ -464,6 +464,7 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
                 for (int w = 0; w < nw; w++) {
                     rec[w] = word(bg.lo, w);
                     rec[nw + w] = word(bg.hi, w);
+                    rec[2 * nw + w] = word(bg.care, w);
                 }
             }
             for (int u = 0; u < ns; u++)
@@ -529,9 +530,9 @@ int encode_query(Query &q, std::string &err) {
     revcomp(rc);
     for (uint32_t k = 0; k < P; k++) {
         uint32_t mm = mismatch_classes(q.pattern[k]);
-        if (mm) { pd.fk[pd.nf] = (uint8_t)k; pd.fm[pd.nf] = (uint8_t)(~mm & 31u); pd.nf++; }
+        if (mm) pd.f[pd.nf++] = k | (~mm & 31u) << 8;
         mm = mismatch_classes(rc[k]);
-        if (mm) { pd.rk[pd.nr] = (uint8_t)k; pd.rm[pd.nr] = (uint8_t)(~mm & 31u); pd.nr++; }
+        if (mm) pd.r[pd.nr++] = k | (~mm & 31u) << 8;
     }
     std::vector<KeyRef> refs;
     for (size_t n = 0; n < q.keys.size(); n++) refs.push_back({&q.keys[n], (uint32_t)n});
