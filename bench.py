@@ -100,8 +100,11 @@ def calibrate_copy(nbytes):
 
 
 def time_steps(ctx, qs, mode, steps, warmup):
-    for _ in range(warmup):
+    tune_ms = 0.0
+    for _ in range(warmup):  # the first batched search runs the autotuner's trials
         ctx.search(qs, mode)
+        tune_ms += ctx.stats()["t_tune_ms"]
+    time_steps.tune_ms = tune_ms
     kernel_ms, launches = 0.0, 0
     t0 = time.perf_counter()
     for _ in range(steps):
@@ -216,6 +219,7 @@ def main():
     log("workload: %s (%d queries, %d guides)" % (args.workload, len(qs), n_guides))
     s_b, k_b, l_b, st_b, r_b = time_steps(ctx, qs, casoff.MODE_BATCHED, args.steps, args.warmup)
     hits_b = r_b.records()
+    tune = {"trial_ms_in_warmup": round(time_steps.tune_ms, 2), "variants": ctx.tuner_info()}
     log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
     if args.no_plain:
         s_p, k_p, l_p, same = float("nan"), float("nan"), 0, None
@@ -258,6 +262,7 @@ def main():
         "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),
                      "base_comparisons": st_b["n_base_cmp"]},
         "genome_generate_upload_s": round(t_load, 2),
+        "autotune": tune,
         "roofline": {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
                      "traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": round(avg_launch_ms, 4),

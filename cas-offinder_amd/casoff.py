@@ -51,6 +51,7 @@ class Stats(C.Structure):
         ("t_collect_ms", C.c_double),
         ("n_loci", C.c_uint64),
         ("n_finder_positions", C.c_uint64),
+        ("t_tune_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -93,6 +94,7 @@ def lib():
             "cof_query_info": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
             "cof_search": (i32, [vp, C.POINTER(vp), i32, i32, C.POINTER(vp)]),
             "cof_stats_get": (i32, [vp, C.POINTER(Stats)]),
+            "cof_tuner_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_result_count": (C.c_int64, [vp]),
             "cof_result_records": (i32, [vp, vp, sz]),
             "cof_result_format": (i32, [vp, vp, i32, C.c_char_p, i32, C.c_char_p]),
@@ -266,6 +268,14 @@ class Context:
         s = Stats()
         lib().cof_stats_get(self._p, C.byref(s))
         return s.as_dict()
+
+    def tuner_info(self):
+        """{launch signature: chosen kernel variant} of the autotuner so far."""
+        out, ln = C.c_void_p(), C.c_size_t()
+        if lib().cof_tuner_info(self._p, C.byref(out), C.byref(ln)):
+            raise CofError(self.error())
+        text = _take_buffer(out, ln.value)
+        return dict((l.rsplit("\t", 1)[0], int(l.rsplit("\t", 1)[1])) for l in text.splitlines() if l)
 
     def format_hits(self, query, hits):
         hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)

@@ -35,6 +35,8 @@ int cof_open(int device, cof_ctx **out) {
     if (hipSetDevice(device) != hipSuccess) return COF_E_DEVICE;
     auto *c = new cof_ctx();
     c->device = device;
+    if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
+    if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = atoi(fv) == 1 ? 1 : 0;
     if (c->lane.init() != hipSuccess) {
         c->lane.release();
         delete c;
@@ -184,6 +186,21 @@ int cof_search(cof_ctx *c, cof_query *const *queries, int n_queries, int mode, c
 int cof_stats_get(const cof_ctx *c, cof_stats *out) {
     if (!c || !out) return COF_E_INVALID;
     *out = c->lane.stats;
+    return COF_OK;
+}
+
+int cof_tuner_info(cof_ctx *c, char **out, size_t *len) {
+    if (!c || !out || !len) return COF_E_INVALID;
+    std::string text;
+    {
+        std::lock_guard<std::mutex> lk(c->tuner.mu);
+        for (auto &kv : c->tuner.best) text += kv.first + "\t" + std::to_string(kv.second) + "\n";
+    }
+    *out = static_cast<char *>(malloc(text.size() + 1));
+    if (!*out) return COF_E_NOMEM;
+    memcpy(*out, text.data(), text.size());
+    (*out)[text.size()] = 0;
+    *len = text.size();
     return COF_OK;
 }
 

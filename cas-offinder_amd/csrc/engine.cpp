@@ -120,6 +120,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
     L.stats.n_launches = launches;
+    L.stats.t_tune_ms = 0;
     L.stats.n_loci = nloci;
     L.stats.n_finder_positions = npos;
     return COF_OK;
@@ -133,7 +134,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
 //    compare keys go into one table (fused_kernel).
 // Records carry global key indices.
 int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
-                   const std::vector<uint32_t> &gbase, uint64_t &nhits, std::string &err) {
+                   const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err) {
     auto group_by = [&](const std::vector<size_t> &idx, auto keyf) {
         std::vector<std::vector<size_t>> groups;
         std::map<std::string, size_t> gi;
@@ -195,20 +196,81 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
     if (kCandBase + nlaunch * kCandWords > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
     auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
+    // launch i < ktab.size(): fused key group i; else bulge group i - ktab.size()
+    auto launch = [&](size_t i, int v, uint32_t stride, uint32_t max_tiles) {
+        if (i < ktab.size())
+            return launch_fused(dg, krep[i]->pat, ktab[i].nw, ktab[i].gen, (uint32_t)ktab[i].cbase,
+                                L.d_keys.p + koff[i], ktab[i].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p,
+                                cand_slots(i), L.d_hkey.n, L.stream, v, stride, max_tiles);
+        const BulgeTable &t = btab[i - ktab.size()];
+        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[i - ktab.size()],
+                            t.units, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i), L.d_hkey.n, L.stream, v,
+                            stride, max_tiles);
+    };
+    auto bucket = [](uint64_t x) {
+        int b = 0;
+        while ((1ull << b) < x) b++;
+        return std::to_string(b);
+    };
+    auto words = [](const void *p, size_t n) {
+        std::string r;
+        const uint32_t *w = static_cast<const uint32_t *>(p);
+        for (size_t k = 0; k < n / 4; k++) r += std::to_string(w[k]) + ",";
+        return r;
+    };
+    std::vector<std::string> sig(nlaunch);
+    std::vector<int> variant(nlaunch, 0);
+    for (size_t i = 0; i < nlaunch; i++) {
+        if (i < ktab.size()) {
+            sig[i] = "fused|" + std::to_string(ktab[i].nw) + (ktab[i].gen ? "g|" : "c|") + krep[i]->pattern + "|" +
+                     bucket(ktab[i].nkeys) + "|" + std::to_string(ktab[i].tmax);
+        } else {
+            const BulgeTable &t = btab[i - ktab.size()];
+            sig[i] = "bulge|" + std::to_string(t.nw) + (t.gen ? "g|" : "c|") + std::to_string(t.ns) +
+                     (t.rev ? "r|" : "f|") + words(t.cb, sizeof t.cb) + "|" + words(t.pat.f, 4 * t.pat.nf) + "|" +
+                     bucket(t.nguides) + "|" + std::to_string(t.tmax);
+        }
+    }
+    double t_tune = 0;
+    if (tuner && tuner->force >= 0) {
+        std::fill(variant.begin(), variant.end(), tuner->force);
+    } else if (tuner && tuner->enabled) {
+        // time every variant of each new signature on tiles spread over the
+        // genome; keep the default unless another is more than 3% faster
+        constexpr uint32_t kTrialTiles = 8192;  // several waves of resident workgroups
+        const uint64_t ntiles = (dg.total + kTilePos - 1) / kTilePos;
+        const uint32_t stride = (uint32_t)std::max<uint64_t>(1, ntiles / kTrialTiles);
+        bool tried = false;
+        const auto tt0 = std::chrono::steady_clock::now();
+        for (size_t i = 0; i < nlaunch; i++) {
+            if (tuner->get(sig[i], variant[i])) continue;
+            const int nv = i < ktab.size() ? kFusedVariants : kBulgeVariants;
+            double best_ms = 0;
+            int best = 0;
+            for (int v = 0; v < nv; v++) {
+                double ms = 1e30;
+                for (int rep = 0; rep < 2; rep++) {
+                    HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
+                    HIP_TRY(launch(i, v, stride, kTrialTiles), "trial launch");
+                    HIP_TRY(hipEventRecord(L.ev[3], L.stream), "hipEventRecord");
+                    HIP_TRY(hipEventSynchronize(L.ev[3]), "trial");
+                    ms = std::min(ms, ms_between(L.ev[2], L.ev[3]));
+                }
+                if (v == 0) best_ms = ms;
+                else if (ms < 0.97 * best_ms) best_ms = ms, best = v;
+            }
+            variant[i] = best;
+            tuner->set(sig[i], best);
+            tried = true;
+        }
+        if (tried)  // trial hits and candidate counts are discarded
+            HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream),
+                    "hipMemsetAsync");
+        t_tune = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
+    }
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-    for (size_t g = 0; g < ktab.size(); g++) {
-        HIP_TRY(launch_fused(dg, krep[g]->pat, ktab[g].nw, ktab[g].gen, (uint32_t)ktab[g].cbase, L.d_keys.p + koff[g],
-                             ktab[g].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(g), L.d_hkey.n,
-                             L.stream),
-                "fused launch");
-    }
-    for (size_t g = 0; g < btab.size(); g++) {
-        const BulgeTable &t = btab[g];
-        HIP_TRY(launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[g], t.units,
-                             L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(ktab.size() + g), L.d_hkey.n,
-                             L.stream),
-                "bulge launch");
-    }
+    for (size_t i = 0; i < nlaunch; i++)
+        HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
     HIP_TRY(hipMemcpyAsync(raw.data(), L.d_count.p, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -246,6 +308,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.n_base_cmp = nbase;
     L.stats.n_model_ops = nmodel;
     L.stats.n_launches = nlaunch;
+    L.stats.t_tune_ms = t_tune;
     L.stats.n_loci = 0;
     L.stats.n_finder_positions = 0;
     nhits = cnt[0];
@@ -285,7 +348,7 @@ void Lane::release() {
 }
 
 int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
-                    int mode, cof_result **out, std::string &err) {
+                    int mode, Tuner *tuner, cof_result **out, std::string &err) {
     *out = nullptr;
     if (qs.size() > 255) { err = "At most 255 queries per search."; return COF_E_INVALID; }
     if (mode != COF_MODE_PLAIN && mode != COF_MODE_BATCHED) { err = "Unknown search mode."; return COF_E_INVALID; }
@@ -306,11 +369,13 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
         HIP_TRY(L.d_hmm.reserve(1 << 20), "hipMalloc(hits)");
     }
     uint64_t nhits = 0;
+    double tune_ms = 0;
     for (int attempt = 0; attempt < 4; attempt++) {
         HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
         int rc = mode == COF_MODE_PLAIN ? search_plain(L, dg, slab_positions, qs, gbase, nhits, err)
-                                        : search_batched(L, dg, qs, gbase, nhits, err);
+                                        : search_batched(L, dg, qs, gbase, tuner, nhits, err);
         if (rc) return rc;
+        tune_ms += L.stats.t_tune_ms;
         if (nhits <= L.d_hkey.n) break;
         const uint64_t want = nhits + nhits / 4 + 1024;  // overflow: grow and rerun
         HIP_TRY(L.d_hkey.reserve(want), "hipMalloc(hits)");
@@ -372,6 +437,7 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     L.stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     L.stats.t_collect_ms = std::chrono::duration<double, std::milli>(t1 - t_kern).count();
     L.stats.n_hits = nhits;
+    L.stats.t_tune_ms = tune_ms;
     L.stats.genome_len = dg.total;
     *out = r;
     return COF_OK;
@@ -420,5 +486,5 @@ int cof_ctx::upload_genome() {
 int cof_ctx::search(const std::vector<std::shared_ptr<Query>> &qs, int mode, cof_result **out) {
     *out = nullptr;
     if (!genome_ready) return fail(COF_E_INVALID, "No genome loaded.");
-    return run_search(lane, dg, slab_positions, qs, mode, out, err);
+    return run_search(lane, dg, slab_positions, qs, mode, &tuner, out, err);
 }

@@ -5,7 +5,9 @@
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
+#include <map>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <utility>
 #include <vector>
@@ -60,13 +62,18 @@ hipError_t launch_finder_write(const DevGenome &g, const PatternDesc &pd, uint64
 hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uint32_t cbase, const uint32_t *table,
                            uint32_t gbase, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
                            uint64_t *hkey, uint16_t *hmm, unsigned long long *count, uint64_t cap, hipStream_t s);
+// variant: fused 0 = 32-key sign blocks, 1 = 8-key blocks; bulge 0 = queued
+// exact counts, 1 = in place.  tile_stride / max_tiles sample tiles (autotuner
+// trials); 1 / 0 scan the whole genome.
+constexpr int kFusedVariants = 2, kBulgeVariants = 2;
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
                         const uint32_t *table, uint32_t nunits, uint32_t gbase, uint64_t *hkey, uint16_t *hmm,
-                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s);
+                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s,
+                        int variant, uint32_t tile_stride, uint32_t max_tiles);
 hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
                         const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
                         uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
-                        hipStream_t s);
+                        hipStream_t s, int variant, uint32_t tile_stride, uint32_t max_tiles);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
                      uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
 
@@ -130,13 +137,39 @@ struct Lane {
     void release();
 };
 
-// Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a
-// COF_* status with the message in err.
 constexpr size_t kCandBase = kSlotStride;
 constexpr size_t kCandWords = (size_t)kCountSlots * kSlotStride;  // per launch
 
+// Autotuner of the batched kernels: the variant of each launch signature
+// (kernel, word count, IUPAC, alignments, pattern, table size bucket, largest
+// threshold) is chosen once per context by timing every variant on a sample
+// of tiles spread over the genome.  Shared by the lanes of a context.
+struct Tuner {
+    std::mutex mu;
+    std::map<std::string, int> best;
+    bool enabled = true;
+    int force = -1;  // >= 0: every launch uses this variant, no trials
+    bool get(const std::string &sig, int &v) {
+        if (force >= 0) {
+            v = force;
+            return true;
+        }
+        std::lock_guard<std::mutex> lk(mu);
+        auto it = best.find(sig);
+        if (it == best.end()) return false;
+        v = it->second;
+        return true;
+    }
+    void set(const std::string &sig, int v) {
+        std::lock_guard<std::mutex> lk(mu);
+        best[sig] = v;
+    }
+};
+
+// Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a
+// COF_* status with the message in err.  tuner may be null (default variants).
 int run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,
-               int mode, cof_result **out, std::string &err);
+               int mode, Tuner *tuner, cof_result **out, std::string &err);
 
 }  // namespace cof
 
@@ -150,6 +183,7 @@ struct cof_ctx {
     cof::DevGenome dg{};
     bool genome_ready = false;
     cof::Lane lane;                        // the context's own lane
+    cof::Tuner tuner;
     uint64_t slab_positions = 1ull << 31;  // plain path window starts per finder launch
 
     int fail(int code, const std::string &m) {

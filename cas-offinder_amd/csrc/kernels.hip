@@ -34,6 +34,8 @@
 #include <rocprim/device/device_radix_sort.hpp>
 #include <rocprim/device/device_scan.hpp>
 
+#include <algorithm>
+
 #include "devtypes.hpp"
 
 namespace cof {
@@ -381,13 +383,6 @@ __device__ __forceinline__ void emit_block(kptr_t kp, uint32_t n, const Score &s
     }
 }
 
-#ifndef COF_BLOCK_KEYS
-#define COF_BLOCK_KEYS 32
-#endif
-#ifndef COF_FUSED_WINDOWS
-#define COF_FUSED_WINDOWS 1
-#endif
-constexpr int kBlockKeys = COF_BLOCK_KEYS;
 
 template <int W, int S, int B, int BLK, typename Score>
 __device__ __forceinline__ bool block_hit(kptr_t kp, const Score &score) {
@@ -405,12 +400,14 @@ __device__ __forceinline__ bool block_hit(kptr_t kp, const Score &score) {
 // Tables of the fused kernel hold a multiple of 8 records per run (padded
 // with records that never hit), so only the plain path's one-key tables reach
 // the per-key tail.
-template <int W, int S, int B, typename Score, typename Sink>
+// BLK keys share one sign test (32 or 8: larger blocks test less often,
+// smaller ones rescore fewer keys when a block holds a hit).
+template <int W, int S, int B, int BLK, typename Score, typename Sink>
 __device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &score, const Sink (&o)[W],
                                           const bool (&valid)[W]) {
     uint32_t k = 0;
-    for (; k + kBlockKeys <= n; k += kBlockKeys, kp += kBlockKeys * S)
-        if (block_hit<W, S, B, kBlockKeys>(kp, score)) emit_block<W, S, B>(kp, kBlockKeys, score, o, valid);
+    for (; k + BLK <= n; k += BLK, kp += BLK * S)
+        if (block_hit<W, S, B, BLK>(kp, score)) emit_block<W, S, B>(kp, BLK, score, o, valid);
     for (; k + 8 <= n; k += 8, kp += 8 * S)
         if (block_hit<W, S, B, 8>(kp, score)) emit_block<W, S, B>(kp, 8, score, o, valid);
     if (k < n) emit_block<W, S, B>(kp, n - k, score, o, valid);
@@ -418,7 +415,7 @@ __device__ __forceinline__ void score_run(kptr_t kp, uint32_t n, const Score &sc
 
 // Scores a whole key table (nunits = care classes for concrete tables, keys
 // for general ones) against W windows (planes l, h, o of each).
-template <int NW, bool GEN, int W, typename Sink>
+template <int NW, bool GEN, int W, int BLK = 32, typename Sink>
 __device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const uint32_t (&l)[W][NW],
                                             const uint32_t (&h)[W][NW], const uint32_t (&o)[W][NW],
                                             const Sink (&sink)[W], const bool (&valid)[W]) {
@@ -435,7 +432,8 @@ __device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const ui
                 cls[w][kClsG][x] = l[w][x] & h[w][x];
                 cls[w][kClsO][x] = o[w][x];
             }
-        score_run<W, S, B>(kp, nunits, [&](kptr_t kw, int w) { return score_general<NW>(kw, cls[w]); }, sink, valid);
+        score_run<W, S, B, BLK>(kp, nunits, [&](kptr_t kw, int w) { return score_general<NW>(kw, cls[w]); }, sink,
+                                valid);
     } else {
         for (uint32_t c = 0; c < nunits; c++) {
             uint32_t lc[W][NW], hc[W][NW], oc[W][NW];
@@ -451,8 +449,8 @@ __device__ __forceinline__ void score_table(kptr_t kp, uint32_t nunits, const ui
             }
             const uint32_t n = kp[NW];
             kp += kClassHdr;
-            score_run<W, S, B>(kp, n, [&](kptr_t kw, int w) { return score_concrete<NW>(kw, lc[w], hc[w], oc[w]); },
-                               sink, valid);
+            score_run<W, S, B, BLK>(
+                kp, n, [&](kptr_t kw, int w) { return score_concrete<NW>(kw, lc[w], hc[w], oc[w]); }, sink, valid);
             kp += n * S;
         }
     }
@@ -617,14 +615,17 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 // With ALL, cands runs on every lane of a wave whose chunk holds a candidate
 // (lanes past the list end get valid = false), so it may use wave-wide
 // ballots and cross-lane queues.
+// The workgroup's tile is blockIdx.x * tile_stride (a stride above 1 samples
+// tiles spread over the genome: autotuner trials).
 template <int W, bool ALL = false, typename Cands>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
-                                                unsigned long long *cand_count, uint32_t *s_lo, uint32_t *s_hi,
-                                                uint32_t *s_ot, uint16_t *s_list, uint32_t *s_wsum, HitStage *st,
-                                                uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
-                                                uint64_t cap, Cands &&cands) {
+                                                uint32_t tile_stride, unsigned long long *cand_count,
+                                                uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot, uint16_t *s_list,
+                                                uint32_t *s_wsum, HitStage *st, uint64_t *hkey, uint16_t *hmm,
+                                                unsigned long long *count, uint64_t cap, Cands &&cands) {
     const int t = threadIdx.x;
-    const int64_t w0 = (int64_t)blockIdx.x * kTileWords;
+    const uint32_t tile = blockIdx.x * tile_stride;
+    const int64_t w0 = (int64_t)tile * kTileWords;
     stage_tile(g, w0, s_lo, s_hi, s_ot);
     if (t == 0) st->n = 0;
     __syncthreads();
@@ -637,7 +638,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
         s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
         keep[j] = range_mask(s0[j], 0, limit);
     }
-    apply_poison(g, pd.P, blockIdx.x, t1, s0, keep);
+    apply_poison(g, pd.P, tile, t1, s0, keep);
     uint32_t mine = 0;
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
@@ -693,7 +694,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
 struct FusedArgs {
     DevGenome g;
     PatternDesc pd;
-    uint32_t cbase, nunits, gbase, pad;
+    uint32_t cbase, nunits, gbase, tile_stride;
     const uint32_t *table;  // key table (nunits care classes, or nunits general records)
     uint64_t limit;         // valid window starts are [0, limit)
     uint64_t *hkey;
@@ -703,16 +704,16 @@ struct FusedArgs {
     uint64_t cap;
 };
 
-template <int NW, bool GEN>
+template <int NW, bool GEN, int BLK>
 __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
     __shared__ HitStage s_stage;
-    constexpr int W = COF_FUSED_WINDOWS;  // candidates per lane (register blocking)
+    constexpr int W = 1;  // candidates per lane
     const uint32_t P = a.pd.P;
-    tile_candidates<W>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey, a.hmm,
-                       a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
+    tile_candidates<W>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage,
+                       a.hkey, a.hmm, a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
                            uint32_t l[W][NW], h[W][NW], o[W][NW];
                            TileSink sink[W];
 #pragma unroll
@@ -725,7 +726,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
                                sink[w] = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase},
                                                   &s_stage, (strand << 15) | ls};
                            }
-                           score_table<NW, GEN, W>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
+                           score_table<NW, GEN, W, BLK>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
                        });
 }
 
@@ -751,6 +752,7 @@ struct BulgeArgs {
     uint32_t gap[kMaxShifts];  // RNA-bulge length of the family of each shift
     uint32_t rev;              // reversed PAM: left part at shift 0, right part at shift u
     uint32_t nunits;           // care classes
+    uint32_t tile_stride;
     const uint32_t *table;
     uint64_t limit;
     uint64_t *hkey;
@@ -861,7 +863,9 @@ __device__ __forceinline__ uint32_t general_mask(kptr_t kp, const uint32_t (&cls
 // mask of passing families << 16.
 constexpr int kBulgeQueue = 64;
 
-template <int NW, bool GEN, int NS>
+// QUEUE = false runs the exact counts in place, for the lanes that passed
+// (fewer registers; cheaper when filters rarely pass).
+template <int NW, bool GEN, int NS, bool QUEUE>
 __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
@@ -872,8 +876,8 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave];
     const kptr_t tab = (kptr_t)a.table;
-    tile_candidates<1, true>(a.g, a.pd, a.limit, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey,
-                             a.hmm, a.count, a.cap,
+    tile_candidates<1, true>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
+                             &s_stage, a.hkey, a.hmm, a.count, a.cap,
                              [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
         // exact counts of the queued pairs, one per lane (records read per lane)
         auto drain = [&](uint32_t qn) {
@@ -979,6 +983,28 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                     acc |= f[u];
                 }
                 const bool pass = valid[0] && acc < 0;
+                if (!QUEUE) {
+                    if (pass) {
+                        const kptr_t vb = tab + kp[KP];
+                        const uint32_t ce = e[0];
+                        const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
+                                            &s_stage, ((ce & 1) << 15) | (ce >> 1)};
+#pragma unroll
+                        for (int u = 0; u < NS; u++) {
+                            if (f[u] < 0) {
+                                const bool lft0 = u == 0 || a.rev;
+                                uint32_t left[NW], right[NW];
+#pragma unroll
+                                for (int w = 0; w < NW; w++) {
+                                    left[w] = lft0 ? M[0][w] : M[u][w];
+                                    right[w] = lft0 ? M[u][w] : M[0][w];
+                                }
+                                bulge_family<NW>(tab, vb + 4 + FB * u, left, right, a.gap[u], kp[KP + 1 + u], sink);
+                            }
+                        }
+                    }
+                    continue;
+                }
                 const uint64_t m = __ballot(pass);
                 if (m) {
                     const uint32_t np = (uint32_t)__popcll(m);
@@ -999,7 +1025,7 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                 }
             }
         }
-        if (qn) drain(qn);
+        if (QUEUE && qn) drain(qn);
     });
 }
 
@@ -1064,17 +1090,35 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
     return hipGetLastError();
 }
 
+// Trial sampling: tile_stride > 1 launches one workgroup per
+// tile_stride-th tile, at most max_tiles of them.
+static dim3 tile_grid(uint64_t limit, uint32_t tile_stride, uint32_t max_tiles) {
+    const uint64_t ntiles = (limit + kTilePos - 1) / kTilePos;
+    uint64_t n = (ntiles + tile_stride - 1) / tile_stride;
+    if (max_tiles) n = std::min<uint64_t>(n, max_tiles);
+    return dim3((uint32_t)n);
+}
+
+template <int NW, bool GEN>
+static void fused_variant(int variant, dim3 grid, const FusedArgs &a, hipStream_t s) {
+    if (variant == 1) hipLaunchKernelGGL((fused_kernel<NW, GEN, 8>), grid, dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL((fused_kernel<NW, GEN, 32>), grid, dim3(kThreads), 0, s, a);
+}
+
 hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, uint32_t cbase,
                         const uint32_t *table, uint32_t nunits, uint32_t gbase, uint64_t *hkey, uint16_t *hmm,
-                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s) {
+                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s,
+                        int variant, uint32_t tile_stride, uint32_t max_tiles) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0 || nunits == 0) return hipSuccess;
+    if (tile_stride == 0) return hipErrorInvalidValue;
     FusedArgs a{};
     a.g = g;
     a.pd = pd;
     a.cbase = cbase;
     a.nunits = nunits;
     a.gbase = gbase;
+    a.tile_stride = tile_stride;
     a.table = table;
     a.limit = limit;
     a.hkey = hkey;
@@ -1082,24 +1126,25 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
-    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos)), block(kThreads);
-    if (nw == 1) {
-        if (gen) hipLaunchKernelGGL((fused_kernel<1, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fused_kernel<1, false>), grid, block, 0, s, a);
-    } else {
-        if (gen) hipLaunchKernelGGL((fused_kernel<2, true>), grid, block, 0, s, a);
-        else hipLaunchKernelGGL((fused_kernel<2, false>), grid, block, 0, s, a);
-    }
+    const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    if (nw == 1) gen ? fused_variant<1, true>(variant, grid, a, s) : fused_variant<1, false>(variant, grid, a, s);
+    else gen ? fused_variant<2, true>(variant, grid, a, s) : fused_variant<2, false>(variant, grid, a, s);
     return hipGetLastError();
 }
 
+template <int NW, bool GEN, int NS>
+static void bulge_variant(int variant, dim3 grid, const BulgeArgs &a, hipStream_t s) {
+    if (variant == 1) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, false>), grid, dim3(kThreads), 0, s, a);
+    else hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, true>), grid, dim3(kThreads), 0, s, a);
+}
+
 template <int NW, bool GEN>
-static hipError_t launch_bulge_ns(int ns, dim3 grid, const BulgeArgs &a, hipStream_t s) {
+static hipError_t launch_bulge_ns(int ns, int variant, dim3 grid, const BulgeArgs &a, hipStream_t s) {
     switch (ns) {
-        case 2: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 2>), grid, dim3(kThreads), 0, s, a); break;
-        case 3: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 3>), grid, dim3(kThreads), 0, s, a); break;
-        case 4: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 4>), grid, dim3(kThreads), 0, s, a); break;
-        case 5: hipLaunchKernelGGL((bulge_kernel<NW, GEN, 5>), grid, dim3(kThreads), 0, s, a); break;
+        case 2: bulge_variant<NW, GEN, 2>(variant, grid, a, s); break;
+        case 3: bulge_variant<NW, GEN, 3>(variant, grid, a, s); break;
+        case 4: bulge_variant<NW, GEN, 4>(variant, grid, a, s); break;
+        case 5: bulge_variant<NW, GEN, 5>(variant, grid, a, s); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -1108,10 +1153,10 @@ static hipError_t launch_bulge_ns(int ns, dim3 grid, const BulgeArgs &a, hipStre
 hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
                         const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
                         uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
-                        hipStream_t s) {
+                        hipStream_t s, int variant, uint32_t tile_stride, uint32_t max_tiles) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0 || nunits == 0) return hipSuccess;
-    if (ns < 2 || ns > kMaxShifts || (nw != 1 && nw != 2)) return hipErrorInvalidValue;
+    if (ns < 2 || ns > kMaxShifts || (nw != 1 && nw != 2) || tile_stride == 0) return hipErrorInvalidValue;
     BulgeArgs a{};
     a.g = g;
     a.pd = pd;
@@ -1121,6 +1166,7 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     }
     a.rev = rev ? 1u : 0u;
     a.nunits = nunits;
+    a.tile_stride = tile_stride;
     a.table = table;
     a.limit = limit;
     a.hkey = hkey;
@@ -1128,9 +1174,10 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
-    const dim3 grid((uint32_t)((limit + kTilePos - 1) / kTilePos));
-    if (nw == 1) return gen ? launch_bulge_ns<1, true>(ns, grid, a, s) : launch_bulge_ns<1, false>(ns, grid, a, s);
-    return gen ? launch_bulge_ns<2, true>(ns, grid, a, s) : launch_bulge_ns<2, false>(ns, grid, a, s);
+    const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    if (nw == 1)
+        return gen ? launch_bulge_ns<1, true>(ns, variant, grid, a, s) : launch_bulge_ns<1, false>(ns, variant, grid, a, s);
+    return gen ? launch_bulge_ns<2, true>(ns, variant, grid, a, s) : launch_bulge_ns<2, false>(ns, variant, grid, a, s);
 }
 
 // Radix sort of the hit records by their 64-bit key (bits [0, end_bit)).

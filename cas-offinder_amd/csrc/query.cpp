@@ -306,6 +306,7 @@ int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, st
     t.cbase = lo;
     t.nw = hi - lo + 1 <= 32 ? 1 : 2;
     t.nkeys = keys.size();
+    for (auto &k : keys) t.tmax = std::max<int>(t.tmax, k.key->threshold);
     encode_records(keys, P, t.cbase, t.nw, t.gen, true, t.words, t.units);
     return 0;
 }
@@ -511,6 +512,8 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
         t.nguides += r.q->bulge.size();
         t.nkeys += r.q->keys.size();
         t.care_sum += r.q->care_sum;
+        for (const BulgeGuide &bg : r.q->bulge)
+            for (int u = 0; u < ns; u++) t.tmax = std::max<int>(t.tmax, bg.tmax[u]);
     }
     return 1;
 }

@@ -92,6 +92,7 @@ struct KeyTable {
     bool gen = false;
     int cbase = 0;
     uint64_t nkeys = 0, care_sum = 0;
+    int tmax = 0;  // largest threshold
 };
 struct KeyRef {
     const CompareKey *key;
@@ -111,6 +112,7 @@ struct BulgeTable {
     bool rev = false;
     PatternDesc pat{};  // finder pattern of the group
     uint64_t nguides = 0, nkeys = 0, care_sum = 0;
+    int tmax = 0;  // largest threshold of an emitting variant
 };
 struct BulgeRef {
     const Query *q;

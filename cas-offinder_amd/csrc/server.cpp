@@ -93,7 +93,7 @@ void cof_server::worker(Lane *L) {
         for (Request *r : batch) qs.insert(qs.end(), r->qs.begin(), r->qs.end());
         cof_result *res = nullptr;
         std::string err;
-        const int rc = run_search(*L, ctx->dg, ctx->slab_positions, qs, COF_MODE_BATCHED, &res, err);
+        const int rc = run_search(*L, ctx->dg, ctx->slab_positions, qs, COF_MODE_BATCHED, &ctx->tuner, &res, err);
         // split the merged result per request, query indices rebased to 0
         std::vector<cof_result *> parts(batch.size(), nullptr);
         if (rc == COF_OK) {

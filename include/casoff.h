@@ -67,6 +67,8 @@ typedef struct {
     double t_collect_ms;    /* device sort of the hits + copy to host + decode */
     uint64_t n_loci;        /* plain mode: (locus, flag) entries the finder wrote */
     uint64_t n_finder_positions; /* plain mode: window starts the finder launches scanned */
+    double t_tune_ms;       /* batched mode: autotuner trials run by this search (first use of a
+                               launch signature on the context; COF_AUTOTUNE=0 disables them) */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference
@@ -135,9 +137,15 @@ int cof_query_info(const cof_query *q, const char **chrdir, uint32_t *pattern_le
  *      part of compareAll (:356-393) for every chunk (main.cpp:58-65).
  *      `queries` run over the resident genome; mode COF_MODE_PLAIN keeps the
  *      reference's launch model (finder, then one comparer launch per key),
- *      COF_MODE_BATCHED runs one fused finder+comparer launch per query. */
+ *      COF_MODE_BATCHED runs one fused finder+comparer launch per pattern
+ *      group of the queries (queries with DNA/RNA bulges: the variants are
+ *      expanded on the device). */
 int cof_search(cof_ctx *ctx, cof_query *const *queries, int n_queries, int mode, cof_result **out);
 int cof_stats_get(const cof_ctx *ctx, cof_stats *out);
+/* Autotuner decisions of the context so far, one "signature<TAB>variant" line
+ * each (no reference counterpart: the reference's launch shape is fixed).
+ * Caller frees *out with cof_free. */
+int cof_tuner_info(cof_ctx *ctx, char **out, size_t *len);
 
 /* ---- results */
 int64_t cof_result_count(const cof_result *r);

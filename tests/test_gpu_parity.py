@@ -15,9 +15,20 @@ pytestmark = pytest.mark.gpu
 MODES = (casoff.MODE_PLAIN, casoff.MODE_BATCHED)
 
 
-@pytest.fixture(scope="module")
-def ctx():
-    c = casoff.Context(0)
+@pytest.fixture(scope="module", params=["tuned", "0", "1"])
+def ctx(request):
+    """A context whose batched launches use the autotuner's variants, or every
+    launch forced to variant 0 / 1 (fused: 32- / 8-key sign blocks; bulge:
+    queued / in-place exact counts), so each kernel variant meets the oracle."""
+    old = os.environ.pop("COF_FORCE_VARIANT", None)
+    if request.param != "tuned":
+        os.environ["COF_FORCE_VARIANT"] = request.param
+    try:
+        c = casoff.Context(0)
+    finally:
+        os.environ.pop("COF_FORCE_VARIANT", None)
+        if old is not None:
+            os.environ["COF_FORCE_VARIANT"] = old
     yield c
     c.close()
 
