@@ -1,13 +1,17 @@
-// Device context: resident genome, work lanes (stream + buffers) and the two
-// search paths (plain: reference launch model; batched: fused per-query launches).
+// Device context: resident genome, work lanes (stream + buffers), the
+// autotuner and the two search paths (plain: reference launch model; batched:
+// one fused launch per pattern group).
 #pragma once
 
 #include <hip/hip_runtime_api.h>
+#include <sys/mman.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <map>
 #include <memory>
 #include <mutex>
+#include <new>
 #include <string>
 #include <utility>
 #include <vector>
@@ -24,6 +28,8 @@ struct cof_query {
 namespace cof {
 // Allocator whose value-less construct leaves trivial elements uninitialized, so
 // resizing a hit vector does not zero-fill memory that is written right after.
+// Blocks of 4 MiB or more are 2 MiB aligned and marked for transparent huge
+// pages: decoding tens of millions of hits then faults in 2 MiB pages.
 template <typename T>
 struct UninitAlloc : std::allocator<T> {
     template <typename U>
@@ -33,6 +39,19 @@ struct UninitAlloc : std::allocator<T> {
     UninitAlloc() = default;
     template <typename U>
     UninitAlloc(const UninitAlloc<U> &) {}
+    static constexpr size_t kHuge = size_t(2) << 20;
+    T *allocate(size_t n) {
+        const size_t bytes = n * sizeof(T);
+        if (bytes < 2 * kHuge) return std::allocator<T>::allocate(n);
+        void *p = std::aligned_alloc(kHuge, (bytes + kHuge - 1) / kHuge * kHuge);
+        if (!p) throw std::bad_alloc();
+        (void)madvise(p, bytes, MADV_HUGEPAGE);
+        return static_cast<T *>(p);
+    }
+    void deallocate(T *p, size_t n) {
+        if (n * sizeof(T) < 2 * kHuge) std::allocator<T>::deallocate(p, n);
+        else std::free(p);
+    }
     template <typename U, typename... A>
     void construct(U *p, A &&...a) {
         if constexpr (sizeof...(A) == 0)
