@@ -219,7 +219,9 @@ def main():
     log("workload: %s (%d queries, %d guides)" % (args.workload, len(qs), n_guides))
     s_b, k_b, l_b, st_b, r_b = time_steps(ctx, qs, casoff.MODE_BATCHED, args.steps, args.warmup)
     hits_b = r_b.records()
-    tune = {"trial_ms_in_warmup": round(time_steps.tune_ms, 2), "variants": ctx.tuner_info()}
+    tune = {"trial_ms_in_warmup": round(time_steps.tune_ms, 2), "variants": ctx.tuner_info(),
+            "timed_steps_launch": {0: "eager launches", 1: "captured HIP graph", 2: "replayed HIP graph"}[
+                int(st_b["graph"])]}
     log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
     if args.no_plain:
         s_p, k_p, l_p, same = float("nan"), float("nan"), 0, None

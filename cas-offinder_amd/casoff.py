@@ -52,6 +52,7 @@ class Stats(C.Structure):
         ("n_loci", C.c_uint64),
         ("n_finder_positions", C.c_uint64),
         ("t_tune_ms", C.c_double),
+        ("graph", C.c_uint64),
     ]
 
     def as_dict(self):

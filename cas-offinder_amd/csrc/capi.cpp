@@ -37,6 +37,7 @@ int cof_open(int device, cof_ctx **out) {
     c->device = device;
     if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
     if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = atoi(fv) == 1 ? 1 : 0;
+    if (const char *gr = getenv("COF_GRAPH")) c->lane.graphs = strcmp(gr, "0") != 0;
     if (c->lane.init() != hipSuccess) {
         c->lane.release();
         delete c;

@@ -121,6 +121,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     L.stats.n_model_ops = nmodel;
     L.stats.n_launches = launches;
     L.stats.t_tune_ms = 0;
+    L.stats.graph = 0;
     L.stats.n_loci = nloci;
     L.stats.n_finder_positions = npos;
     return COF_OK;
@@ -190,9 +191,35 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         kw.resize((kw.size() + 3) & ~(size_t)3, 0);
     }
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
-    if (!kw.empty())
-        HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
+    // identity of the launch sequence: every kernel argument and the tables
+    std::string gkey;
+    if (L.graphs) {
+        auto put = [&](const void *p, size_t n) { gkey.append(static_cast<const char *>(p), n); };
+        const void *ptrs[] = {L.d_keys.p, L.d_hkey.p, L.d_hmm.p, L.d_count.p, dg.lo, dg.poison};
+        put(ptrs, sizeof ptrs);
+        put(&L.d_hkey.n, sizeof L.d_hkey.n);
+        put(&dg.total, sizeof dg.total);
+        for (size_t g = 0; g < ktab.size(); g++) {
+            const int v[4] = {ktab[g].nw, ktab[g].gen, ktab[g].cbase, (int)ktab[g].units};
+            put(v, sizeof v);
+            put(&krep[g]->pat, sizeof(PatternDesc));
+            put(&koff[g], sizeof koff[g]);
+        }
+        for (size_t g = 0; g < btab.size(); g++) {
+            const BulgeTable &t = btab[g];
+            const int v[5] = {t.nw, t.gen, t.ns, t.rev, (int)t.units};
+            put(v, sizeof v);
+            put(t.cb, sizeof t.cb);
+            put(t.gap, sizeof t.gap);
+            put(&t.pat, sizeof(PatternDesc));
+            put(&boff[g], sizeof boff[g]);
+        }
+        put(kw.data(), kw.size() * 4);
+    }
+    const bool replay = L.graph && L.graphs && gkey == L.graph_key;
+    if (!replay && !kw.empty())
+        HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
     if (kCandBase + nlaunch * kCandWords > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
     auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
@@ -232,7 +259,9 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         }
     }
     double t_tune = 0;
-    if (tuner && tuner->force >= 0) {
+    if (replay) {
+        variant = L.graph_variants;
+    } else if (tuner && tuner->force >= 0) {
         std::fill(variant.begin(), variant.end(), tuner->force);
     } else if (tuner && tuner->enabled) {
         // time every variant of each new signature on tiles spread over the
@@ -269,8 +298,39 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         t_tune = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
     }
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-    for (size_t i = 0; i < nlaunch; i++)
-        HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+    if (!replay && L.graphs && nlaunch) {
+        // capture the launch sequence (kernels only: counters are cleared
+        // before, outside the graph); a failed capture falls back to eager
+        // launches for the rest of the lane's life
+        if (L.graph) (void)hipGraphExecDestroy(L.graph);
+        L.graph = nullptr;
+        L.graph_key.clear();
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal);
+        for (size_t i = 0; e == hipSuccess && i < nlaunch; i++) e = launch(i, variant[i], 1, 0);
+        hipError_t e2 = hipStreamEndCapture(L.stream, &g);
+        if (e == hipSuccess && e2 == hipSuccess && g)
+            e = hipGraphInstantiate(&L.graph, g, nullptr, nullptr, 0);
+        else if (e == hipSuccess)
+            e = e2 != hipSuccess ? e2 : hipErrorUnknown;
+        if (g) (void)hipGraphDestroy(g);
+        if (e == hipSuccess) {
+            L.graph_key = gkey;
+            L.graph_variants = variant;
+        } else {
+            (void)hipGetLastError();
+            L.graph = nullptr;
+            L.graphs = false;
+        }
+    }
+    L.stats.graph = 0;
+    if (L.graph && L.graphs && L.graph_key == gkey) {
+        HIP_TRY(hipGraphLaunch(L.graph, L.stream), "hipGraphLaunch");
+        L.stats.graph = replay ? 2 : 1;
+    } else {
+        for (size_t i = 0; i < nlaunch; i++)
+            HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+    }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
     HIP_TRY(hipMemcpyAsync(raw.data(), L.d_count.p, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -327,6 +387,9 @@ hipError_t Lane::init() {
 
 void Lane::release() {
     if (stream) (void)hipStreamSynchronize(stream);
+    if (graph) (void)hipGraphExecDestroy(graph);
+    graph = nullptr;
+    graph_key.clear();
     d_hkey.release();
     d_hkey2.release();
     d_hmm.release();

@@ -152,6 +152,12 @@ struct Lane {
     DevBuf<uint32_t> d_keys;
     hipEvent_t ev[6] = {};
     cof_stats stats{};
+    // batched launch sequence of the last search as a HIP graph, replayed when
+    // a search has the same tables, kernel arguments and buffers
+    hipGraphExec_t graph = nullptr;
+    std::string graph_key;
+    std::vector<int> graph_variants;
+    bool graphs = true;
     hipError_t init();
     void release();
 };

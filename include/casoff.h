@@ -69,6 +69,8 @@ typedef struct {
     uint64_t n_finder_positions; /* plain mode: window starts the finder launches scanned */
     double t_tune_ms;       /* batched mode: autotuner trials run by this search (first use of a
                                launch signature on the context; COF_AUTOTUNE=0 disables them) */
+    uint64_t graph;         /* batched mode: 0 launches issued one by one, 1 captured as a HIP
+                               graph and launched, 2 the previous search's graph replayed */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference

@@ -1,0 +1,58 @@
+"""HIP graph replay of the batched launch sequence: every replay, with other
+GPU work and synchronizations between replays, gives the records of eager
+launches (a context with COF_GRAPH=0)."""
+import os
+
+import pytest
+
+import casoff
+import workloads
+
+pytestmark = pytest.mark.gpu
+
+
+def _context(graph):
+    old = os.environ.pop("COF_GRAPH", None)
+    if not graph:
+        os.environ["COF_GRAPH"] = "0"
+    try:
+        return casoff.Context(0)
+    finally:
+        os.environ.pop("COF_GRAPH", None)
+        if old is not None:
+            os.environ["COF_GRAPH"] = old
+
+
+def test_graph_replays_match_eager(tmp_path):
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=21, n_contigs=5, total_len=3_000_000, n_frac=0.01), gp, 0)
+    texts = [t for _, t in workloads.group_inputs(workloads.mixed_guides(4, 60, 3, 6, 0.5, 1), gp)]
+    eager, graph = _context(False), _context(True)
+    try:
+        for c in (eager, graph):
+            c.load(gp)
+        qe = [eager.query(t) for t in texts]
+        qg = [graph.query(t) for t in texts]
+        want_all = eager.search(qe, casoff.MODE_BATCHED).records()
+        assert eager.stats()["graph"] == 0
+        want_one = eager.search(qe[:1], casoff.MODE_BATCHED).records()
+        assert len(want_all) > 0
+        modes = []
+        for rep in range(4):
+            got = graph.search(qg, casoff.MODE_BATCHED).records()
+            modes.append(graph.stats()["graph"])
+            assert len(got) == len(want_all) and (got == want_all).all(), "replay %d" % rep
+            # other GPU work and synchronizations between replays
+            assert (eager.search(qe[:1], casoff.MODE_BATCHED).records() == want_one).all()
+            graph.text(0, 100)
+        assert modes[0] == 1 and all(m == 2 for m in modes[1:]), modes
+        # a different search recaptures; returning to the first recaptures again
+        got = graph.search(qg[:1], casoff.MODE_BATCHED).records()
+        assert graph.stats()["graph"] == 1 and (got == want_one).all()
+        got = graph.search(qg, casoff.MODE_BATCHED).records()
+        assert graph.stats()["graph"] == 1 and (got == want_all).all()
+        got = graph.search(qg, casoff.MODE_BATCHED).records()
+        assert graph.stats()["graph"] == 2 and (got == want_all).all()
+    finally:
+        eager.close()
+        graph.close()
