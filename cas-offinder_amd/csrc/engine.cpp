@@ -263,7 +263,9 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         variant = L.graph_variants;
     } else if (tuner && tuner->force >= 0) {
         std::fill(variant.begin(), variant.end(), tuner->force);
-    } else if (tuner && tuner->enabled) {
+    } else if (tuner && !(tuner->enabled && L.trials)) {
+        for (size_t i = 0; i < nlaunch; i++) (void)tuner->get(sig[i], variant[i]);  // default when untuned
+    } else if (tuner) {
         // time every variant of each new signature on tiles spread over the
         // genome; keep the default unless another is more than 3% faster
         constexpr uint32_t kTrialTiles = 8192;  // several waves of resident workgroups

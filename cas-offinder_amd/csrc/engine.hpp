@@ -158,6 +158,7 @@ struct Lane {
     std::string graph_key;
     std::vector<int> graph_variants;
     bool graphs = true;
+    bool trials = true;  // may run autotuner trials (off where latency matters: serving)
     hipError_t init();
     void release();
 };

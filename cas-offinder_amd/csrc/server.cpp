@@ -151,6 +151,7 @@ int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out) 
     for (int i = 0; i < nlanes; i++) {
         auto L = std::make_unique<Lane>();
         L->graphs = false;  // merged batches differ from call to call: nothing to replay
+        L->trials = false;  // tuned variants of the context are used, untuned ones default
         if (L->init() != hipSuccess) {
             for (auto &x : s->lanes) x->release();
             L->release();
