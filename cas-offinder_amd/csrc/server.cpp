@@ -152,7 +152,13 @@ int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out) 
         auto L = std::make_unique<Lane>();
         L->graphs = false;  // merged batches differ from call to call: nothing to replay
         L->trials = false;  // tuned variants of the context are used, untuned ones default
-        if (L->init() != hipSuccess) {
+        // hit buffers sized up front: a first large request then does not pay
+        // for an overflow rerun
+        constexpr uint64_t kLaneHits = 16ull << 20;
+        if (L->init() != hipSuccess || L->d_hkey.reserve(kLaneHits) != hipSuccess ||
+            L->d_hmm.reserve(kLaneHits) != hipSuccess || L->d_hkey2.reserve(kLaneHits) != hipSuccess ||
+            L->d_hmm2.reserve(kLaneHits) != hipSuccess || L->h_hkey.reserve(kLaneHits) != hipSuccess ||
+            L->h_hmm.reserve(kLaneHits) != hipSuccess) {
             for (auto &x : s->lanes) x->release();
             L->release();
             delete s;
