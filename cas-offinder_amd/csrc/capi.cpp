@@ -1,4 +1,5 @@
 // extern "C" entry points of libcasoff.so (declared in include/casoff.h).
+#include <algorithm>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
@@ -36,7 +37,7 @@ int cof_open(int device, cof_ctx **out) {
     auto *c = new cof_ctx();
     c->device = device;
     if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
-    if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = atoi(fv) == 1 ? 1 : 0;
+    if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = std::min(std::max(atoi(fv), 0), 2);
     if (const char *gr = getenv("COF_GRAPH")) c->lane.graphs = strcmp(gr, "0") != 0;
     if (c->lane.init() != hipSuccess) {
         c->lane.release();

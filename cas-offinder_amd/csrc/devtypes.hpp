@@ -68,6 +68,23 @@ constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that neve
 constexpr int kCountSlots = 64;
 constexpr int kSlotStride = 16;
 
+// Seed index of a single-class concrete key table (fused_seed_kernel): the
+// comparable positions are split into `parts` = largest threshold + 1 bit
+// ranges, so a window within threshold of a key matches at least one range of
+// the key exactly (pigeonhole).  Per range, offs[obase + code .. + 1) delimit
+// the ids of the records whose bases there (2-bit codes, lo bits then hi bits)
+// equal `code`; a key with N inside a range is listed under every code it
+// matches.
+constexpr int kSeedMaxParts = 8;
+struct SeedDesc {
+    uint32_t parts;
+    uint32_t nrec;                  // records copied to LDS (12 bytes each: klo, khi, bias)
+    uint32_t noffs, nids;           // uint16 entries of the offset and id arrays
+    uint32_t first[kSeedMaxParts];  // bit of the range's first position | length << 8
+    uint32_t mask[kSeedMaxParts];   // the range's bits
+    uint32_t obase[kSeedMaxParts];  // the range's first entry in offs
+};
+
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.
 constexpr int kHitKeyShift = 40;
 constexpr uint64_t kMaxGenomePositions = 1ull << (kHitKeyShift - 1);

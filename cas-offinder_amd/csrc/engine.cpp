@@ -177,13 +177,27 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         if (build_key_table(refs, qs[kgroups[g][0]]->P, ktab[g], err)) return COF_E_INVALID;
         krep[g] = qs[kgroups[g][0]].get();
     }
-    // one device buffer: key tables, then bulge tables
+    // seed indexes of the key tables that qualify (an autotuner variant)
+    std::vector<SeedIndex> seeds(ktab.size());
+    std::vector<char> has_seed(ktab.size(), 0);
+    for (size_t g = 0; g < ktab.size(); g++) {
+        std::vector<KeyRef> refs;
+        for (size_t qi : kgroups[g])
+            for (size_t k = 0; k < qs[qi]->keys.size(); k++) refs.push_back({&qs[qi]->keys[k], gbase[qi] + (uint32_t)k});
+        has_seed[g] = build_seed_index(refs, krep[g]->P, ktab[g], seeds[g]);
+    }
+    // one device buffer: key tables (each followed by its seed index), then bulge tables
     std::vector<uint32_t> kw;
-    std::vector<size_t> koff(ktab.size()), boff(btab.size());
+    std::vector<size_t> koff(ktab.size()), soff(ktab.size(), 0), boff(btab.size());
     for (size_t g = 0; g < ktab.size(); g++) {
         koff[g] = kw.size();
         kw.insert(kw.end(), ktab[g].words.begin(), ktab[g].words.end());
         kw.resize((kw.size() + 3) & ~(size_t)3, 0);  // 16-byte aligned tables
+        if (has_seed[g]) {
+            soff[g] = kw.size();
+            kw.insert(kw.end(), seeds[g].words.begin(), seeds[g].words.end());
+            kw.resize((kw.size() + 3) & ~(size_t)3, 0);
+        }
     }
     for (size_t g = 0; g < btab.size(); g++) {
         boff[g] = kw.size();
@@ -205,6 +219,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(v, sizeof v);
             put(&krep[g]->pat, sizeof(PatternDesc));
             put(&koff[g], sizeof koff[g]);
+            put(&soff[g], sizeof soff[g]);
         }
         for (size_t g = 0; g < btab.size(); g++) {
             const BulgeTable &t = btab[g];
@@ -225,6 +240,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
     // launch i < ktab.size(): fused key group i; else bulge group i - ktab.size()
     auto launch = [&](size_t i, int v, uint32_t stride, uint32_t max_tiles) {
+        if (i < ktab.size() && v == 2 && has_seed[i])
+            return launch_seed(dg, krep[i]->pat, seeds[i].desc, seeds[i].lds_bytes, (uint32_t)ktab[i].cbase,
+                               L.d_keys.p + koff[i], L.d_keys.p + soff[i], L.d_hkey.p, L.d_hmm.p, L.d_count.p,
+                               cand_slots(i), L.d_hkey.n, L.stream, stride, max_tiles);
         if (i < ktab.size())
             return launch_fused(dg, krep[i]->pat, ktab[i].nw, ktab[i].gen, (uint32_t)ktab[i].cbase,
                                 L.d_keys.p + koff[i], ktab[i].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p,
@@ -262,7 +281,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     if (replay) {
         variant = L.graph_variants;
     } else if (tuner && tuner->force >= 0) {
-        std::fill(variant.begin(), variant.end(), tuner->force);
+        for (size_t i = 0; i < nlaunch; i++)
+            variant[i] = (i < ktab.size() && tuner->force == 2 && !has_seed[i]) ? 0 : std::min(tuner->force, 1 + (i < ktab.size()));
     } else if (tuner && !(tuner->enabled && L.trials)) {
         for (size_t i = 0; i < nlaunch; i++) (void)tuner->get(sig[i], variant[i]);  // default when untuned
     } else if (tuner) {
@@ -275,7 +295,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         const auto tt0 = std::chrono::steady_clock::now();
         for (size_t i = 0; i < nlaunch; i++) {
             if (tuner->get(sig[i], variant[i])) continue;
-            const int nv = i < ktab.size() ? kFusedVariants : kBulgeVariants;
+            const int nv = i < ktab.size() ? (has_seed[i] ? kFusedVariants : 2) : kBulgeVariants;
             double best_ms = 0;
             int best = 0;
             for (int v = 0; v < nv; v++) {

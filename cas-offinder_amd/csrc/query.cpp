@@ -311,6 +311,83 @@ int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, st
     return 0;
 }
 
+bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTable &t, SeedIndex &s) {
+    (void)P;
+    s = SeedIndex();
+    if (t.gen || t.nw != 1 || t.units != 1 || keys.empty() || keys.size() > 4096) return false;
+    const uint32_t care = t.words[0];
+    const int C = __builtin_popcount(care);
+    const int parts = t.tmax + 1;
+    if (parts > kSeedMaxParts || parts > C) return false;
+    // ranges: consecutive groups of comparable positions, as equal as possible
+    std::vector<int> pos;
+    for (int b = 0; b < 32; b++)
+        if ((care >> b) & 1) pos.push_back(b);
+    SeedDesc &d = s.desc;
+    d.parts = (uint32_t)parts;
+    d.nrec = (uint32_t)keys.size();
+    uint32_t obase = 0;
+    for (int p = 0; p < parts; p++) {
+        const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
+        const int a = pos[g0], z = g1 < C ? pos[g1] : pos[C - 1] + 1;  // bits [a, z)
+        const int len = z - a;
+        if (len > 5) return false;  // at most 1024 codes per range
+        d.first[p] = (uint32_t)a | (uint32_t)len << 8;
+        d.mask[p] = ((1u << len) - 1) << a;
+        d.obase[p] = obase;
+        obase += (1u << (2 * len)) + 1;
+    }
+    d.noffs = obase;
+    // (code, record) pairs per range, keys expanded over their N positions
+    std::vector<std::vector<std::pair<uint32_t, uint16_t>>> ent(parts);
+    for (size_t n = 0; n < keys.size(); n++) {
+        const std::string &k = keys[n].key->seq;
+        for (int p = 0; p < parts; p++) {
+            const int a = (int)(d.first[p] & 0xFF), len = (int)(d.first[p] >> 8);
+            std::vector<uint32_t> cs = {0};
+            for (int i = 0; i < len; i++) {
+                const char c = k[(size_t)(t.cbase + a + i)];
+                std::vector<int> bases;
+                if (c == 'T') bases = {0};
+                else if (c == 'C') bases = {1};
+                else if (c == 'A') bases = {2};
+                else if (c == 'G') bases = {3};
+                else bases = {0, 1, 2, 3};  // N (not comparable): any window base
+                std::vector<uint32_t> nx;
+                for (uint32_t c0 : cs)
+                    for (int b : bases) nx.push_back(c0 | (uint32_t)(b & 1) << i | (uint32_t)(b >> 1) << (len + i));
+                cs.swap(nx);
+                if (cs.size() > 64) return false;
+            }
+            for (uint32_t c : cs) ent[p].push_back({c, (uint16_t)n});
+        }
+    }
+    std::vector<uint32_t> offs(obase, 0);
+    std::vector<uint16_t> ids;
+    for (int p = 0; p < parts; p++) {
+        std::stable_sort(ent[p].begin(), ent[p].end(),
+                         [](const std::pair<uint32_t, uint16_t> &x, const std::pair<uint32_t, uint16_t> &y) {
+                             return x.first < y.first;
+                         });
+        const uint32_t ncode = 1u << (2 * (d.first[p] >> 8));
+        size_t e = 0;
+        for (uint32_t c = 0; c <= ncode; c++) {
+            offs[d.obase[p] + c] = (uint32_t)ids.size();
+            while (c < ncode && e < ent[p].size() && ent[p][e].first == c) ids.push_back(ent[p][e++].second);
+        }
+    }
+    if (ids.size() > 0xFFFF) return false;
+    d.nids = (uint32_t)ids.size();
+    std::vector<uint16_t> all;
+    for (uint32_t o : offs) all.push_back((uint16_t)o);
+    all.insert(all.end(), ids.begin(), ids.end());
+    if (all.size() & 1) all.push_back(0);
+    s.words.resize(all.size() / 2);
+    memcpy(s.words.data(), all.data(), all.size() * 2);
+    s.lds_bytes = d.nrec * 12 + (uint32_t)all.size() * 2;
+    return s.lds_bytes <= 24 * 1024;
+}
+
 namespace {
 
 // Bulge families of every guide of q (see BulgeGuide).  Each compare key is

@@ -100,6 +100,17 @@ struct KeyRef {
 };
 int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, std::string &err);
 
+// Seed index of a key table built by build_key_table from `keys`: words =
+// offs then ids (uint16 pairs packed in uint32).  False when the table does not qualify (IUPAC
+// codes, two words, several care classes, thresholds too high for the
+// comparable positions, or an index over the LDS budget).
+struct SeedIndex {
+    SeedDesc desc{};
+    std::vector<uint32_t> words;
+    uint32_t lds_bytes = 0;
+};
+bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTable &t, SeedIndex &s);
+
 // Bulge-family table of queries sharing padded pattern, D, R and PAM side.
 struct BulgeTable {
     std::vector<uint32_t> words;

@@ -15,11 +15,12 @@ pytestmark = pytest.mark.gpu
 MODES = (casoff.MODE_PLAIN, casoff.MODE_BATCHED)
 
 
-@pytest.fixture(scope="module", params=["tuned", "0", "1"])
+@pytest.fixture(scope="module", params=["tuned", "0", "1", "2"])
 def ctx(request):
     """A context whose batched launches use the autotuner's variants, or every
-    launch forced to variant 0 / 1 (fused: 32- / 8-key sign blocks; bulge:
-    queued / in-place exact counts), so each kernel variant meets the oracle."""
+    launch forced to variant 0 / 1 / 2 (fused: 32- / 8-key sign blocks / seed
+    index where the table qualifies; bulge: queued / in-place exact counts), so
+    each kernel variant meets the oracle."""
     old = os.environ.pop("COF_FORCE_VARIANT", None)
     if request.param != "tuned":
         os.environ["COF_FORCE_VARIANT"] = request.param
