@@ -60,15 +60,17 @@ def build_workload(name, ctx, n_guides=None):
     return texts, qs, guides, desc
 
 
-def kernels_of(texts):
-    """Kernels the batched path launches for these inputs: bulge_kernel for
-    queries with DNA/RNA bulges (D + R <= 4), fused_kernel for the others."""
+def kernels_of(variants):
+    """Kernels the batched path launched, from the autotuner's decisions
+    (signature -> variant): fused_kernel or fused_seed_kernel (variant 2) for
+    key groups, bulge_kernel for groups with DNA/RNA bulges."""
     names = set()
-    for t in texts:
-        tk = t.split("\n")[1].split()
-        bulged = len(tk) == 3 and 0 < int(tk[1]) + int(tk[2]) <= 4
-        names.add("bulge_kernel" if bulged else "fused_kernel")
-    return "+".join(sorted(names, reverse=True))
+    for sig, v in variants.items():
+        if sig.startswith("bulge|"):
+            names.add("bulge_kernel")
+        else:
+            names.add("fused_seed_kernel" if v == 2 else "fused_kernel")
+    return "+".join(sorted(names))
 
 
 def measured_traffic(workload):
@@ -240,7 +242,7 @@ def main():
     survey = SURVEY_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
     hbm_achieved = 0.375 * total / launch_s / 1e9
     traffic, traffic_src = measured_traffic(args.workload)
-    kernel = kernels_of(texts)
+    kernel = kernels_of(tune["variants"])
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -270,8 +272,11 @@ def main():
                      "traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": round(avg_launch_ms, 4),
                      "work_per_launch": "%d int32 ops (%s)" % (st_b["n_model_ops"] / max(l_b, 1), {
                          "fused_kernel": "3 per (candidate, key, 32-position word)",
+                         "fused_seed_kernel": "3 per (candidate, key, 32-position word) of the brute-force "
+                                              "compare; the seed index counts only the keys that match a range "
+                                              "exactly, so this is brute-force-equivalent work",
                          "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
-                     }.get(kernel, "fused_kernel and bulge_kernel units summed")),
+                     }.get(kernel, "units of the kernels summed")),
                      "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
                                            "frac": round(survey / VALU_PEAK_TOPS, 3)},
                      "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
