@@ -237,3 +237,28 @@ def test_bulge_families(ctx, tmp_path, case):
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "bulge case %d" % case)
     assert len(exp) > 0
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_seed_index_shared_inner_n(ctx, tmp_path, seed):
+    """Guides with N at the same inner positions form one care class whose seed
+    index lists a key under every code of a range holding its N; thresholds
+    0-5 give 1-6 ranges."""
+    r = random.Random(500 + seed)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(30000, 80000), "ACGTN" if i == 0 else "ACGT", n_runs=2))
+            for i in range(3)]
+    gp = str(tmp_path / "s.fa")
+    H.write_fasta(gp, recs)
+    pattern = "N" * 20 + "NGG"
+    holes = r.sample(range(1, 19), r.randint(1, 3))
+    guides = []
+    for core in _planted_guides(r, recs, pattern, 20, 8):
+        g = list(core)
+        for p in holes:
+            g[p] = "N"
+        guides.append("".join(g) + "NNN")
+    tmax = r.randint(0, 5)
+    lines = [pattern] + ["%s %d" % (g, r.randint(0, tmax)) for g in guides]
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "seed %d" % seed)
+    assert len(exp) > 0
