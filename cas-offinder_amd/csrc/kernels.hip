@@ -231,39 +231,52 @@ __device__ __forceinline__ void emit_hit(const HitSink &o, uint32_t key, uint32_
 // pass instead of one per hit (same-address atomics serialize in one L2
 // channel).  An entry packs global key index (bits 23..46), mismatches
 // (16..22), strand (15) and the start's offset in the tile (0..14).
-constexpr int kStageCap = 512;
-struct HitStage {
-    uint64_t ent[kStageCap];
-    uint32_t n;  // entries appended this pass; past kStageCap they went to HBM directly
+struct HitStage {  // header of a workgroup's stage
+    uint32_t n;       // entries appended this pass; past the capacity they went to HBM directly
     uint32_t pad;
     unsigned long long base;
 };
+template <int CAP>
+struct HitStageBuf {
+    HitStage h;
+    uint64_t ent[CAP];
+};
+struct StageRef {
+    HitStage *h;
+    uint64_t *ent;
+    uint32_t cap;
+};
+template <int CAP>
+__device__ __forceinline__ StageRef stage_ref(HitStageBuf<CAP> &b) {
+    return StageRef{&b.h, b.ent, (uint32_t)CAP};
+}
+constexpr int kStageCap = 512;  // default stage entries
 static_assert(kTilePos <= (1 << 15) && kMaxPattern < (1 << 7), "stage entry fields");
 
 struct TileSink {
     HitSink g;      // direct path when the stage is full
-    HitStage *st;
+    StageRef st;
     uint32_t low;   // strand << 15 | tile offset
 };
 
 __device__ __forceinline__ void emit_hit(const TileSink &o, uint32_t key, uint32_t mm) {
-    const uint32_t i = atomicAdd(&o.st->n, 1u);
-    if (i < (uint32_t)kStageCap)
-        o.st->ent[i] = ((uint64_t)(o.g.gbase + key) << 23) | (mm << 16) | o.low;
+    const uint32_t i = atomicAdd(&o.st.h->n, 1u);
+    if (i < o.st.cap)
+        o.st.ent[i] = ((uint64_t)(o.g.gbase + key) << 23) | (mm << 16) | o.low;
     else
         emit_hit(o.g, key, mm);
 }
 
 // Writes the staged hits of the pass (all threads, after a barrier) and
 // empties the stage.
-__device__ __forceinline__ void flush_stage(HitStage *st, uint64_t t0, uint64_t *hkey, uint16_t *hmm,
+__device__ __forceinline__ void flush_stage(StageRef st, uint64_t t0, uint64_t *hkey, uint16_t *hmm,
                                             unsigned long long *count, uint64_t cap) {
-    const uint32_t n = min(st->n, (uint32_t)kStageCap);
-    if (threadIdx.x == 0) st->base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
+    const uint32_t n = min(st.h->n, st.cap);
+    if (threadIdx.x == 0) st.h->base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
     __syncthreads();
-    const unsigned long long base = st->base;
+    const unsigned long long base = st.h->base;
     for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
-        const uint64_t e = st->ent[i];
+        const uint64_t e = st.ent[i];
         const unsigned long long o = base + i;
         if (o < cap) {
             hkey[o] = ((e >> 23) << kHitKeyShift) | ((t0 + (e & 0x7FFF)) << 1) | ((e >> 15) & 1);
@@ -271,7 +284,7 @@ __device__ __forceinline__ void flush_stage(HitStage *st, uint64_t t0, uint64_t 
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) st->n = 0;
+    if (threadIdx.x == 0) st.h->n = 0;
 }
 
 // NW words of the window planes starting at plane bit `bit` (rd(i) reads word
@@ -621,13 +634,13 @@ template <int W, bool ALL = false, int LC = kListCap, typename Cands>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 uint32_t tile_stride, unsigned long long *cand_count,
                                                 uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot, uint16_t *s_list,
-                                                uint32_t *s_wsum, HitStage *st, uint64_t *hkey, uint16_t *hmm,
+                                                uint32_t *s_wsum, StageRef st, uint64_t *hkey, uint16_t *hmm,
                                                 unsigned long long *count, uint64_t cap, Cands &&cands) {
     const int t = threadIdx.x;
     const uint32_t tile = blockIdx.x * tile_stride;
     const int64_t w0 = (int64_t)tile * kTileWords;
     stage_tile(g, w0, s_lo, s_hi, s_ot);
-    if (t == 0) st->n = 0;
+    if (t == 0) st.h->n = 0;
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
     uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
@@ -709,11 +722,12 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
-    __shared__ HitStage s_stage;
+    __shared__ HitStageBuf<kStageCap> s_stage;
     constexpr int W = 1;  // candidates per lane
     const uint32_t P = a.pd.P;
-    tile_candidates<W>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage,
-                       a.hkey, a.hmm, a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
+    tile_candidates<W>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
+                       stage_ref(s_stage), a.hkey, a.hmm, a.count, a.cap,
+                       [&](uint64_t t0, auto rd, const uint32_t (&e)[W], const bool (&valid)[W]) {
                            uint32_t l[W][NW], h[W][NW], o[W][NW];
                            TileSink sink[W];
 #pragma unroll
@@ -724,7 +738,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
                                                                      32 * kHaloL;
                                window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
                                sink[w] = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase},
-                                                  &s_stage, (strand << 15) | ls};
+                                                  stage_ref(s_stage), (strand << 15) | ls};
                            }
                            score_table<NW, GEN, W, BLK>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
                        });
@@ -755,13 +769,14 @@ struct SeedArgs {
     uint64_t cap;
 };
 
-constexpr int kSeedListCap = 2048;  // smaller candidate list: LDS room for the index
+constexpr int kSeedListCap = 2048;  // smaller candidate list and hit stage: LDS room for the index
+constexpr int kSeedStageCap = 128;
 
 __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kSeedListCap];
     __shared__ uint32_t s_wsum[4];
-    __shared__ HitStage s_stage;
+    __shared__ HitStageBuf<kSeedStageCap> s_stage;
     extern __shared__ __align__(16) uint32_t s_seed[];  // records (3 words each), offs, ids
     const SeedDesc &sd = a.sd;
     uint32_t *s_rec = s_seed;
@@ -780,15 +795,15 @@ __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
     const uint32_t care = a.table[0];
     const uint32_t P = a.pd.P;
     tile_candidates<1, false, kSeedListCap>(
-        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, &s_stage, a.hkey, a.hmm,
-        a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
+        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, stage_ref(s_stage), a.hkey,
+        a.hmm, a.count, a.cap, [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
             const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
             const int64_t bit = strand == 0 ? (int64_t)ls + a.cbase + 32 * kHaloL
                                             : (int64_t)ls + P - 32 - (int64_t)a.cbase + 32 * kHaloL;
             uint32_t l[1], h[1], o[1];
             window_planes<1>(rd, bit, strand, l, h, o);
             const uint32_t lc = l[0] & care, hc = h[0] & care, oc = o[0] & care;
-            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, &s_stage,
+            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(s_stage),
                                 (strand << 15) | ls};
             for (uint32_t p = 0; p < sd.parts; p++) {
                 const uint32_t pm = sd.mask[p];
@@ -955,14 +970,14 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
-    __shared__ HitStage s_stage;
+    __shared__ HitStageBuf<kStageCap> s_stage;
     __shared__ uint32_t s_qrec[4][kBulgeQueue], s_qinfo[4][kBulgeQueue];
     constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS), FB = bulge_family_words(NW);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave];
     const kptr_t tab = (kptr_t)a.table;
     tile_candidates<1, true>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
-                             &s_stage, a.hkey, a.hmm, a.count, a.cap,
+                             stage_ref(s_stage), a.hkey, a.hmm, a.count, a.cap,
                              [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
         // exact counts of the queued pairs, one per lane (records read per lane)
         auto drain = [&](uint32_t qn) {
@@ -993,8 +1008,8 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                     }
                 }
                 const kptr_t vb = tab + kp[KP];
-                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]}, &s_stage,
-                                    ((ce & 1) << 15) | (ce >> 1)};
+                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
+                                    stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if ((fam >> u) & 1) {
@@ -1073,7 +1088,7 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                         const kptr_t vb = tab + kp[KP];
                         const uint32_t ce = e[0];
                         const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
-                                            &s_stage, ((ce & 1) << 15) | (ce >> 1)};
+                                            stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
 #pragma unroll
                         for (int u = 0; u < NS; u++) {
                             if (f[u] < 0) {
@@ -1102,7 +1117,8 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
 #pragma unroll
                         for (int u = 0; u < NS; u++) fam |= (f[u] < 0 ? 1u : 0u) << u;
                         const uint32_t slot =
-                            qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                            qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32),
+                                                           __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                         qrec[slot] = (uint32_t)(kp - tab);
                         qinfo[slot] = e[0] | (fam << 16);
                     }
@@ -1287,7 +1303,8 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
     a.cap = cap;
     const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
     if (nw == 1)
-        return gen ? launch_bulge_ns<1, true>(ns, variant, grid, a, s) : launch_bulge_ns<1, false>(ns, variant, grid, a, s);
+        return gen ? launch_bulge_ns<1, true>(ns, variant, grid, a, s)
+                   : launch_bulge_ns<1, false>(ns, variant, grid, a, s);
     return gen ? launch_bulge_ns<2, true>(ns, variant, grid, a, s) : launch_bulge_ns<2, false>(ns, variant, grid, a, s);
 }
 
