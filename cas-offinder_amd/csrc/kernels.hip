@@ -127,11 +127,13 @@ __device__ __forceinline__ uint32_t range_mask(uint64_t s0, uint64_t lo, uint64_
 // Clears, in the masks of the words starting at s0[j], the starts whose window
 // [s, s+P) holds a poison position.  The per-tile table gives the first poison
 // at or after the tile start t0, so a tile without one costs one scalar load.
+// ppos = g.tile_ppos[tile], read by the caller before staging the tile so
+// that the two memory latencies overlap.
 template <int NJ>
 __device__ __forceinline__ void apply_poison(const DevGenome &g, uint32_t P, uint64_t tile, uint64_t t1,
-                                             const uint64_t (&s0)[NJ], uint32_t (&keep)[NJ]) {
+                                             uint64_t ppos, const uint64_t (&s0)[NJ], uint32_t (&keep)[NJ]) {
     const uint64_t end = t1 + P - 1;  // poisons at or past `end` touch no start of this tile
-    if (g.tile_ppos[tile] >= end) return;
+    if (ppos >= end) return;
     for (uint64_t i = g.tile_pidx[tile];; i++) {
         const uint64_t x = g.poison[i];  // poison[npoison] is the sentinel
         if (x >= end) break;
@@ -495,6 +497,7 @@ __device__ __forceinline__ uint64_t slab_tile_masks(const FinderArgs &a, uint32_
                                                     uint32_t &nmine) {
     const int t = threadIdx.x;
     const int64_t w0 = (int64_t)(a.slab_lo >> 5) + (int64_t)blockIdx.x * kTileWords;
+    const uint64_t ppos = a.g.tile_ppos[(uint64_t)w0 * 32 / kTilePos];
     stage_tile(a.g, w0, s_lo, s_hi, s_ot);
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
@@ -506,7 +509,7 @@ __device__ __forceinline__ uint64_t slab_tile_masks(const FinderArgs &a, uint32_
         s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
         keep[j] = range_mask(s0[j], a.slab_lo, a.slab_hi);
     }
-    apply_poison(a.g, a.pd.P, t0 / kTilePos, t1, s0, keep);
+    apply_poison(a.g, a.pd.P, t0 / kTilePos, t1, ppos, s0, keep);
     mine = nmine = 0;
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
@@ -639,6 +642,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     const int t = threadIdx.x;
     const uint32_t tile = blockIdx.x * tile_stride;
     const int64_t w0 = (int64_t)tile * kTileWords;
+    const uint64_t ppos = g.tile_ppos[tile];
     stage_tile(g, w0, s_lo, s_hi, s_ot);
     if (t == 0) st.h->n = 0;
     __syncthreads();
@@ -651,7 +655,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
         s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
         keep[j] = range_mask(s0[j], 0, limit);
     }
-    apply_poison(g, pd.P, tile, t1, s0, keep);
+    apply_poison(g, pd.P, tile, t1, ppos, s0, keep);
     uint32_t mine = 0;
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
