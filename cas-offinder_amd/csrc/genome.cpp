@@ -91,6 +91,68 @@ void HostGenome::append(const char *s, size_t n) {
     }
 }
 
+void HostGenome::append_parallel(const char *s, size_t n) {
+    const int nt = n_host_threads();
+    if (n < (size_t(1) << 22) || nt <= 1) {
+        append(s, n);
+        return;
+    }
+    size_t i = 0;
+    while (i < n && (total & 31)) append(s + i++, 1);  // up to a word boundary
+    const uint64_t nw = (n - i) / 32, w0 = total >> 5;
+    lo.resize(w0 + nw);
+    hi.resize(w0 + nw);
+    ot.resize(w0 + nw);
+    const char *body = s + i;
+    const uint8_t *tab = code_table();
+    struct Part {
+        std::vector<ExcRun> exc;
+        std::vector<uint64_t> poison;
+    };
+    std::vector<Part> parts((size_t)nt);
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; t++) {
+        th.emplace_back([&, t] {
+            Part &pt = parts[(size_t)t];
+            const uint64_t wb = nw * (uint64_t)t / (uint64_t)nt, we = nw * (uint64_t)(t + 1) / (uint64_t)nt;
+            for (uint64_t w = wb; w < we; w++) {
+                uint32_t l = 0, h = 0, o = 0;
+                const char *c = body + 32 * w;
+                for (uint32_t b = 0; b < 32; b++) {
+                    const uint8_t code = tab[(unsigned char)c[b]];
+                    if (code < 4) {
+                        l |= (uint32_t)(code & 1) << b;
+                        h |= (uint32_t)(code >> 1) << b;
+                    } else {
+                        o |= 1u << b;
+                        const uint64_t pos = total + 32 * w + b;
+                        if (!pt.exc.empty() && pt.exc.back().ch == c[b] && pt.exc.back().start + pt.exc.back().len == pos)
+                            pt.exc.back().len++;
+                        else
+                            pt.exc.push_back({pos, 1, c[b]});
+                        if (c[b] == ';') pt.poison.push_back(pos);
+                    }
+                }
+                lo[w0 + w] = l;
+                hi[w0 + w] = h;
+                ot[w0 + w] = o;
+            }
+        });
+    }
+    for (auto &x : th) x.join();
+    for (Part &pt : parts) {  // chunks in position order: runs may continue across chunks
+        for (const ExcRun &r : pt.exc) {
+            if (!exc.empty() && exc.back().ch == r.ch && exc.back().start + exc.back().len == r.start)
+                exc.back().len += r.len;
+            else
+                exc.push_back(r);
+        }
+        poison.insert(poison.end(), pt.poison.begin(), pt.poison.end());
+    }
+    total += 32 * nw;
+    append(body + 32 * nw, n - i - 32 * nw);
+}
+
 void HostGenome::begin_file(const std::string &path) {
     if (!files.empty()) append_char(';');  // separator between files
     GenomeFile f;
@@ -155,48 +217,94 @@ int load_fasta(const std::string &path, HostGenome &g, std::string &err) {
     if (!f) return 1;
     int c0 = fgetc(f);
     if (c0 != '>') { fclose(f); return 1; }
+    struct stat stt;
+    if (fstat(fileno(f), &stt) != 0) { fclose(f); err = "cannot stat " + path; return -1; }
+    const size_t size = (size_t)stt.st_size;
+    std::vector<char> file(size);
     rewind(f);
-    g.begin_file(path);
-    GenomeFile *gf = &g.files.back();
-    const size_t kBuf = 1 << 24;
-    std::vector<char> buf(kBuf);
-    std::string carry, line;
-    bool have = false;
-    auto handle_line = [&](const char *s, size_t n) {
-        if (n == 0) return;
-        if (s[n - 1] == '\r') n--;
-        if (n > 0 && s[0] == '>') {
-            if (have) g.append_char(';');
-            gf->chrpos.push_back(g.total - gf->base);
-            gf->names.emplace_back(s + 1, n - 1);
-            have = true;
-        } else {
-            line.assign(s, n);
-            for (auto &ch : line)
-                if (ch >= 'a' && ch <= 'z') ch = (char)(ch - 32);
-            g.append(line.data(), line.size());
+    if (size && fread(file.data(), 1, size, f) != size) { fclose(f); err = "read error: " + path; return -1; }
+    fclose(f);
+    // chunks that start at line starts; pass 1 counts each chunk's output
+    // bytes (sequence bytes and the ';' before every record but the first)
+    const int nt = std::max(1, std::min<int>(n_host_threads(), (int)(size >> 20) + 1));
+    std::vector<size_t> cb((size_t)nt + 1, size);
+    cb[0] = 0;
+    for (int t = 1; t < nt; t++) {
+        size_t b = size * (size_t)t / (size_t)nt;
+        const char *nl = b < size ? static_cast<const char *>(memchr(file.data() + b, '\n', size - b)) : nullptr;
+        cb[(size_t)t] = nl ? (size_t)(nl - file.data()) + 1 : size;
+    }
+    for (int t = 1; t <= nt; t++) cb[(size_t)t] = std::max(cb[(size_t)t], cb[(size_t)t - 1]);
+    struct Rec {
+        uint64_t out;  // output offset (in the chunk) where the record's sequence starts
+        std::string name;
+    };
+    struct Chunk {
+        uint64_t out = 0;
+        std::vector<Rec> recs;
+    };
+    std::vector<Chunk> ch((size_t)nt);
+    // for_lines(t, fn): fn(line start, length without '\n' and trailing '\r')
+    auto for_lines = [&](int t, auto &&fn) {
+        size_t p = cb[(size_t)t];
+        const size_t e = cb[(size_t)t + 1];
+        while (p < e) {
+            const char *nl = static_cast<const char *>(memchr(file.data() + p, '\n', e - p));
+            const size_t le = nl ? (size_t)(nl - file.data()) : e;
+            size_t n = le - p;
+            if (n > 0 && file[le - 1] == '\r') n--;
+            if (n > 0) fn(p, n);
+            p = le + 1;
         }
     };
-    size_t r;
-    while ((r = fread(buf.data(), 1, kBuf, f)) > 0) {
-        size_t s = 0;
-        for (size_t i = 0; i < r; i++) {
-            if (buf[i] == '\n') {
-                if (!carry.empty()) {
-                    carry.append(buf.data() + s, i - s);
-                    handle_line(carry.data(), carry.size());
-                    carry.clear();
-                } else {
-                    handle_line(buf.data() + s, i - s);
-                }
-                s = i + 1;
-            }
-        }
-        carry.append(buf.data() + s, r - s);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t] {
+                Chunk &c = ch[(size_t)t];
+                for_lines(t, [&](size_t p, size_t n) {
+                    if (file[p] == '>') {
+                        if (p != 0) c.out++;  // ';' before every record but the first
+                        c.recs.push_back({c.out, std::string(file.data() + p + 1, n - 1)});
+                    } else {
+                        c.out += n;
+                    }
+                });
+            });
+        for (auto &x : th) x.join();
     }
-    if (!carry.empty()) handle_line(carry.data(), carry.size());
-    if (ferror(f)) { err = "read error: " + path; fclose(f); g.end_file(); return -1; }
-    fclose(f);
+    std::vector<uint64_t> base((size_t)nt + 1, 0);
+    for (int t = 0; t < nt; t++) base[(size_t)t + 1] = base[(size_t)t] + ch[(size_t)t].out;
+    // pass 2: uppercased sequence bytes and separators, at the chunks' offsets
+    std::vector<char> text(base[(size_t)nt]);
+    {
+        std::vector<std::thread> th;
+        for (int t = 0; t < nt; t++)
+            th.emplace_back([&, t] {
+                char *o = text.data() + base[(size_t)t];
+                for_lines(t, [&](size_t p, size_t n) {
+                    if (file[p] == '>') {
+                        if (p != 0) *o++ = ';';
+                    } else {
+                        for (size_t k = 0; k < n; k++) {
+                            const char c = file[p + k];
+                            *o++ = (c >= 'a' && c <= 'z') ? (char)(c - 32) : c;
+                        }
+                    }
+                });
+            });
+        for (auto &x : th) x.join();
+    }
+    std::vector<char>().swap(file);
+    g.begin_file(path);
+    GenomeFile &gf = g.files.back();
+    const uint64_t off = g.total - gf.base;  // file text offset of text[0]
+    for (int t = 0; t < nt; t++)
+        for (Rec &r : ch[(size_t)t].recs) {
+            gf.chrpos.push_back(off + base[(size_t)t] + r.out);
+            gf.names.push_back(std::move(r.name));
+        }
+    g.append_parallel(text.data(), text.size());
     g.end_file();
     return 0;
 }
@@ -250,7 +358,7 @@ int load_twobit(const std::string &path, HostGenome &g, std::string &err) {
         }
         gf->chrpos.push_back(g.total - gf->base);
         gf->names.push_back(names[i]);
-        g.append(seq.data(), seq.size());
+        g.append_parallel(seq.data(), seq.size());
         if (i + 1 < cnt) g.append_char(';');
     }
     fclose(f);

@@ -58,6 +58,7 @@ public:
     void begin_file(const std::string &path);
     void end_file();
     void append(const char *s, size_t n);  // bytes already uppercased
+    void append_parallel(const char *s, size_t n);  // same result, whole words packed by host threads
     void append_char(char c) { append(&c, 1); }
     void finish();  // pads the planes
 

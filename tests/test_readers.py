@@ -134,3 +134,48 @@ def test_missing_path_is_an_error(tmp_path):
     c = casoff.Context(-1)
     with pytest.raises(casoff.CofError):
         c.load(str(tmp_path / "nope"))
+
+
+def _messy_fasta(path, seed, n_records, approx_len):
+    """A FASTA file of several MB (the loaders split it into per-thread
+    chunks): mixed LF/CRLF line ends, empty lines, lowercase, non-ACGT bytes,
+    empty records, N runs and ';' bytes inside sequences."""
+    r = H.rng(seed)
+    parts = []
+    for i in range(n_records):
+        eol = "\r\n" if r.random() < 0.3 else "\n"
+        parts.append(">rec%d %s%s" % (i, "x" * r.randint(0, 5), eol))
+        n = 0 if r.random() < 0.05 else r.randint(1, 2 * approx_len // n_records)
+        seq = list(r.choices("ACGTacgtACGTacgtNnRY-;", k=n))
+        for _ in range(n // 5000):
+            st, run = r.randrange(n), r.randint(30, 300)
+            seq[st:st + run] = "N" * len(seq[st:st + run])
+        seq = "".join(seq)
+        pos = 0
+        while pos < len(seq):
+            w = r.choice([1, 31, 32, 33, 60, 61, 80, 997])
+            parts.append(seq[pos:pos + w] + eol)
+            if r.random() < 0.02:
+                parts.append(eol)
+            pos += w
+    with open(path, "w", newline="") as f:
+        f.write("".join(parts))
+
+
+@pytest.mark.parametrize("seed", range(2))
+def test_large_fasta_threaded_chunks(tmp_path, seed):
+    p = str(tmp_path / "big.fa")
+    _messy_fasta(p, seed, n_records=40 + 200 * seed, approx_len=6_000_000)
+    assert os.path.getsize(p) > 5_000_000
+    check_all(p)
+
+
+def test_large_2bit_and_fasta_agree(tmp_path):
+    spec = casoff.synth_spec(seed=7, n_contigs=5, total_len=9_000_001, n_frac=0.04, n_run_min=1,
+                             n_run_max=3000, lower_frac=0.5, line_width=61)
+    fa, tb = str(tmp_path / "g.fa"), str(tmp_path / "g.2bit")
+    casoff.synth_write(spec, fa, 0)
+    casoff.synth_write(spec, tb, 1)
+    a = check_all(fa)
+    b = check_all(tb)
+    assert a == b
