@@ -62,14 +62,15 @@ def build_workload(name, ctx, n_guides=None):
 
 def kernels_of(variants):
     """Kernels the batched path launched, from the autotuner's decisions
-    (signature -> variant): fused_kernel or fused_seed_kernel (variant 2) for
-    key groups, bulge_kernel for groups with DNA/RNA bulges."""
+    (signature -> variant): fused_kernel, fused_seed_kernel (variant 2) or
+    fused_mask_kernel (variants 3, 4) for key groups, bulge_kernel for groups
+    with DNA/RNA bulges."""
     names = set()
     for sig, v in variants.items():
         if sig.startswith("bulge|"):
             names.add("bulge_kernel")
         else:
-            names.add("fused_seed_kernel" if v == 2 else "fused_kernel")
+            names.add({2: "fused_seed_kernel", 3: "fused_mask_kernel", 4: "fused_mask_kernel"}.get(v, "fused_kernel"))
     return "+".join(sorted(names))
 
 
@@ -275,6 +276,9 @@ def main():
                          "fused_seed_kernel": "3 per (candidate, key, 32-position word) of the brute-force "
                                               "compare; the seed index counts only the keys that match a range "
                                               "exactly, so this is brute-force-equivalent work",
+                         "fused_mask_kernel": "3 per (candidate, key, 32-position word) of the brute-force "
+                                              "compare; the bit-set seed index counts only the keys that match "
+                                              "enough ranges exactly, so this is brute-force-equivalent work",
                          "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
                      }.get(kernel, "units of the kernels summed")),
                      "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),

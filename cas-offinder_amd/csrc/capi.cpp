@@ -37,7 +37,7 @@ int cof_open(int device, cof_ctx **out) {
     auto *c = new cof_ctx();
     c->device = device;
     if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
-    if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = std::min(std::max(atoi(fv), 0), 2);
+    if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = std::min(std::max(atoi(fv), 0), cof::kFusedVariants - 1);
     if (const char *gr = getenv("COF_GRAPH")) c->lane.graphs = strcmp(gr, "0") != 0;
     if (c->lane.init() != hipSuccess) {
         c->lane.release();

@@ -85,6 +85,25 @@ struct SeedDesc {
     uint32_t obase[kSeedMaxParts];  // the range's first entry in offs
 };
 
+// Bit-set seed index of a single-class concrete key table of at most
+// 32 * kMaskMaxWords keys (fused_mask_kernel).  The comparable positions are
+// split into `parts` bit ranges of at most kMaskMaxSpan bits; per range and
+// 2-bit code (lo bits then hi bits) a row of nk words has bit r set when
+// record r's bases there equal the code (a key with N inside a range is set
+// under every code).  A window within threshold t of a key matches at least
+// parts - t of the key's ranges exactly; the kernel counts only the records
+// that match at least `minex` ranges (parts >= largest threshold + minex).
+// Layout (words, copied to LDS as is): rows (row i at i * nk; row zrow is all
+// zero: a range holding a genome N at a comparable position matches
+// nothing), then at recw the records [klo, khi, bias, key index].
+constexpr int kMaskMaxWords = 8;
+constexpr int kMaskMaxSpan = 4;
+struct MaskDesc {
+    uint32_t parts, nk, minex, nrec;
+    uint32_t zrow, recw, nwords, care;  // nwords: rows and records, a multiple of 4
+    uint32_t first[kSeedMaxParts];  // bit of the range's first position | span << 8 | first row << 16
+};
+
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.
 constexpr int kHitKeyShift = 40;
 constexpr uint64_t kMaxGenomePositions = 1ull << (kHitKeyShift - 1);

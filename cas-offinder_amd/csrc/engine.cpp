@@ -180,15 +180,19 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // seed indexes of the key tables that qualify (an autotuner variant)
     std::vector<SeedIndex> seeds(ktab.size());
     std::vector<char> has_seed(ktab.size(), 0);
+    std::vector<MaskIndex> masks[2] = {std::vector<MaskIndex>(ktab.size()), std::vector<MaskIndex>(ktab.size())};
+    std::vector<char> has_mask[2] = {std::vector<char>(ktab.size(), 0), std::vector<char>(ktab.size(), 0)};
     for (size_t g = 0; g < ktab.size(); g++) {
         std::vector<KeyRef> refs;
         for (size_t qi : kgroups[g])
             for (size_t k = 0; k < qs[qi]->keys.size(); k++) refs.push_back({&qs[qi]->keys[k], gbase[qi] + (uint32_t)k});
         has_seed[g] = build_seed_index(refs, krep[g]->P, ktab[g], seeds[g]);
+        for (int x = 0; x < 2; x++) has_mask[x][g] = build_mask_index(refs, ktab[g], x + 1, masks[x][g]);
     }
     // one device buffer: key tables (each followed by its seed index), then bulge tables
     std::vector<uint32_t> kw;
     std::vector<size_t> koff(ktab.size()), soff(ktab.size(), 0), boff(btab.size());
+    std::vector<size_t> moff[2] = {std::vector<size_t>(ktab.size(), 0), std::vector<size_t>(ktab.size(), 0)};
     for (size_t g = 0; g < ktab.size(); g++) {
         koff[g] = kw.size();
         kw.insert(kw.end(), ktab[g].words.begin(), ktab[g].words.end());
@@ -198,6 +202,12 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             kw.insert(kw.end(), seeds[g].words.begin(), seeds[g].words.end());
             kw.resize((kw.size() + 3) & ~(size_t)3, 0);
         }
+        for (int x = 0; x < 2; x++)
+            if (has_mask[x][g]) {
+                moff[x][g] = kw.size();
+                kw.insert(kw.end(), masks[x][g].words.begin(), masks[x][g].words.end());
+                kw.resize((kw.size() + 3) & ~(size_t)3, 0);
+            }
     }
     for (size_t g = 0; g < btab.size(); g++) {
         boff[g] = kw.size();
@@ -220,6 +230,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(&krep[g]->pat, sizeof(PatternDesc));
             put(&koff[g], sizeof koff[g]);
             put(&soff[g], sizeof soff[g]);
+            put(&moff[0][g], sizeof moff[0][g]);
+            put(&moff[1][g], sizeof moff[1][g]);
         }
         for (size_t g = 0; g < btab.size(); g++) {
             const BulgeTable &t = btab[g];
@@ -240,6 +252,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
     // launch i < ktab.size(): fused key group i; else bulge group i - ktab.size()
     auto launch = [&](size_t i, int v, uint32_t stride, uint32_t max_tiles) {
+        if (i < ktab.size() && (v == 3 || v == 4) && has_mask[v - 3][i])
+            return launch_mask(dg, krep[i]->pat, masks[v - 3][i].desc, (uint32_t)ktab[i].cbase,
+                               L.d_keys.p + moff[v - 3][i], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
+                               L.d_hkey.n, L.stream, stride, max_tiles);
         if (i < ktab.size() && v == 2 && has_seed[i])
             return launch_seed(dg, krep[i]->pat, seeds[i].desc, seeds[i].lds_bytes, (uint32_t)ktab[i].cbase,
                                L.d_keys.p + koff[i], L.d_keys.p + soff[i], L.d_hkey.p, L.d_hmm.p, L.d_count.p,
@@ -277,14 +293,26 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                      bucket(t.nguides) + "|" + std::to_string(t.tmax);
         }
     }
+    // variants launch i can run (a cached choice for the same signature may
+    // name an index the table does not qualify for: then the default runs)
+    auto avail = [&](size_t i, int v) {
+        if (i >= ktab.size()) return v >= 0 && v < kBulgeVariants;
+        switch (v) {
+            case 0: case 1: return true;
+            case 2: return (bool)has_seed[i];
+            case 3: case 4: return (bool)has_mask[v - 3][i];
+            default: return false;
+        }
+    };
     double t_tune = 0;
     if (replay) {
         variant = L.graph_variants;
     } else if (tuner && tuner->force >= 0) {
         for (size_t i = 0; i < nlaunch; i++)
-            variant[i] = (i < ktab.size() && tuner->force == 2 && !has_seed[i]) ? 0 : std::min(tuner->force, 1 + (i < ktab.size()));
+            variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : kBulgeVariants - 1;
     } else if (tuner && !(tuner->enabled && L.trials)) {
-        for (size_t i = 0; i < nlaunch; i++) (void)tuner->get(sig[i], variant[i]);  // default when untuned
+        for (size_t i = 0; i < nlaunch; i++)  // default when untuned
+            if (!tuner->get(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
     } else if (tuner) {
         // time every variant of each new signature on tiles spread over the
         // genome; keep the default unless another is more than 3% faster
@@ -294,11 +322,12 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         bool tried = false;
         const auto tt0 = std::chrono::steady_clock::now();
         for (size_t i = 0; i < nlaunch; i++) {
-            if (tuner->get(sig[i], variant[i])) continue;
-            const int nv = i < ktab.size() ? (has_seed[i] ? kFusedVariants : 2) : kBulgeVariants;
+            if (tuner->get(sig[i], variant[i]) && avail(i, variant[i])) continue;
+            const int nv = i < ktab.size() ? kFusedVariants : kBulgeVariants;
             double best_ms = 0;
             int best = 0;
             for (int v = 0; v < nv; v++) {
+                if (!avail(i, v)) continue;
                 double ms = 1e30;
                 for (int rep = 0; rep < 2; rep++) {
                     HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");

@@ -82,10 +82,16 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
                            uint32_t gbase, uint64_t slab_lo, const uint32_t *loci, const uint8_t *flags, uint64_t n,
                            uint64_t *hkey, uint16_t *hmm, unsigned long long *count, uint64_t cap, hipStream_t s);
 // variant: fused 0 = 32-key sign blocks, 1 = 8-key blocks, 2 = seed index
-// (launch_seed; single-class concrete tables); bulge 0 = queued exact counts,
-// 1 = in place.  tile_stride / max_tiles sample tiles (autotuner trials); 1 / 0
-// scan the whole genome.
-constexpr int kFusedVariants = 3, kBulgeVariants = 2;
+// (launch_seed; single-class concrete tables), 3 / 4 = bit-set seed index
+// keeping records that match 1 / 2 ranges exactly (launch_mask; single-class
+// concrete tables of at most 256 keys); bulge 0 = queued exact counts, 1 = in
+// place.  tile_stride / max_tiles sample tiles (autotuner trials); 1 / 0 scan
+// the whole genome.
+constexpr int kFusedVariants = 5, kBulgeVariants = 2;
+hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
+                       const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                       unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
+                       uint32_t max_tiles);
 hipError_t launch_seed(const DevGenome &g, const PatternDesc &pd, const SeedDesc &sd, uint32_t lds_bytes,
                        uint32_t cbase, const uint32_t *table, const uint32_t *seed, uint64_t *hkey, uint16_t *hmm,
                        unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s,

@@ -676,17 +676,21 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     for (uint32_t pass = 0; pass < n; pass += LC) {
         const uint32_t cnt = min(n - pass, (uint32_t)LC);
         if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
-            const bool all = n <= (uint32_t)LC;
-            uint32_t idx = first;
+            // entry of bit b of word wi: (wi * 32 + b) << 1 | strand; idx counts
+            // from the pass start (it wraps below it: then idx >= cnt)
+            uint32_t idx = first - pass;
+            auto put = [&](bool all) {
 #pragma unroll
-            for (int j = 0; j < kWordsPerThread; j++) {
-                const uint32_t wi = (uint32_t)(kWordsPerThread * t + j);
-                for (uint64_t m = (uint64_t)fwd[j] | ((uint64_t)rev[j] << 32); m; m &= m - 1, idx++) {
-                    const uint32_t b = (uint32_t)__builtin_ctzll(m);
-                    if (all || (idx >= pass && idx < pass + cnt))
-                        s_list[idx - pass] = (uint16_t)(((wi * 32 + (b & 31)) << 1) | (b >> 5));
+                for (int j = 0; j < kWordsPerThread; j++) {
+                    const uint32_t base = (uint32_t)(kWordsPerThread * t + j) << 6;
+                    for (uint32_t m = fwd[j]; m; m &= m - 1, idx++)
+                        if (all || idx < cnt) s_list[idx] = (uint16_t)(base + 2 * __builtin_ctz(m));
+                    for (uint32_t m = rev[j]; m; m &= m - 1, idx++)
+                        if (all || idx < cnt) s_list[idx] = (uint16_t)(base + 2 * __builtin_ctz(m) + 1);
                 }
-            }
+            };
+            if (n <= (uint32_t)LC) put(true);  // one pass: every entry is in it
+            else put(false);
         }
         __syncthreads();
         for (uint32_t base = (uint32_t)wave * 64 * W; base < cnt; base += kThreads * W) {
@@ -829,6 +833,158 @@ __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
                             emit_hit(sink, a.table[kClassHdr + 4 * r + 3],
                                      (uint32_t)(sc - (int32_t)s_rec[3 * r + 2]));
                     }
+                }
+            }
+        });
+}
+
+// fused_mask_kernel: a single-class concrete key table of at most 32 * NK
+// records with a bit-set seed index (MaskDesc).  Per candidate, each range's
+// 2-bit codes select a row of NK words (bit r: record r's bases there equal
+// the window's); the rows of all ranges are combined into the records that
+// match at least MINEX ranges exactly (OR for 1; for 2, bits seen twice), and
+// only those records are counted, each once.  A window within threshold t of
+// a key matches at least parts - t >= MINEX of its ranges, so no hit is
+// missed.  Compared with fused_seed_kernel there are no per-range lists to
+// walk: the row loads are independent and the per-lane loop runs over the
+// few surviving records.
+
+struct MaskArgs {
+    DevGenome g;
+    PatternDesc pd;
+    MaskDesc md;
+    uint32_t cbase, tile_stride;
+    const uint32_t *idx;  // rows then records (md.nwords words)
+    uint64_t limit;
+    uint64_t *hkey;
+    uint16_t *hmm;
+    unsigned long long *count;
+    unsigned long long *cand_count;
+    uint64_t cap;
+};
+
+template <int NK>
+__device__ __forceinline__ void load_row(const uint32_t *s_rows, uint32_t row, uint32_t (&m)[NK]) {
+    if constexpr (NK >= 4) {
+#pragma unroll
+        for (int q = 0; q < NK / 4; q++) {
+            const uint4 v = reinterpret_cast<const uint4 *>(s_rows + row * NK)[q];
+            m[4 * q] = v.x;
+            m[4 * q + 1] = v.y;
+            m[4 * q + 2] = v.z;
+            m[4 * q + 3] = v.w;
+        }
+    } else if constexpr (NK == 2) {
+        const uint2 v = reinterpret_cast<const uint2 *>(s_rows)[row];
+        m[0] = v.x;
+        m[1] = v.y;
+    } else {
+        m[0] = s_rows[row];
+    }
+}
+
+constexpr uint32_t kOr3 = 0xFE;  // a | b | c
+constexpr uint32_t kMaj = 0xE8;  // at least two of a, b, c
+
+// Bits set in at least one (any) and at least two (two) of the NP rows, in
+// groups of three: a bit is in two rows when it is in two rows of one group
+// or in two groups.
+template <int NP, int NK, bool TWO>
+__device__ __forceinline__ void combine_rows(const uint32_t (&m)[NP][NK], uint32_t (&any)[NK], uint32_t (&two)[NK]) {
+    constexpr int G = (NP + 2) / 3;
+    static_assert(G <= 3, "at most nine rows");
+#pragma unroll
+    for (int w = 0; w < NK; w++) {
+        uint32_t g1[G], g2[G];
+#pragma unroll
+        for (int g = 0; g < G; g++) {
+            const int n = NP - 3 * g < 3 ? NP - 3 * g : 3;
+            if (n == 3) {
+                g1[g] = __builtin_amdgcn_bitop3_b32(m[3 * g][w], m[3 * g + 1][w], m[3 * g + 2][w], kOr3);
+                g2[g] = __builtin_amdgcn_bitop3_b32(m[3 * g][w], m[3 * g + 1][w], m[3 * g + 2][w], kMaj);
+            } else if (n == 2) {
+                g1[g] = m[3 * g][w] | m[3 * g + 1][w];
+                g2[g] = m[3 * g][w] & m[3 * g + 1][w];
+            } else {
+                g1[g] = m[3 * g][w];
+                g2[g] = 0;
+            }
+        }
+        if (G == 1) {
+            any[w] = g1[0];
+            two[w] = g2[0];
+        } else if (G == 2) {
+            any[w] = g1[0] | g1[1];
+            if (TWO) two[w] = __builtin_amdgcn_bitop3_b32(g1[0], g1[1], g2[0] | g2[1], kAndOr);
+        } else {
+            any[w] = __builtin_amdgcn_bitop3_b32(g1[0], g1[1], g1[2], kOr3);
+            if (TWO)
+                two[w] = __builtin_amdgcn_bitop3_b32(g2[0], g2[1], g2[2], kOr3) |
+                         __builtin_amdgcn_bitop3_b32(g1[0], g1[1], g1[2], kMaj);
+        }
+    }
+}
+
+// One-word window planes read from the staged tile (LDS word index and
+// shift in 32-bit arithmetic); reverse complemented for strand 1.
+__device__ __forceinline__ void window_lds(const uint32_t *s_lo, const uint32_t *s_hi, const uint32_t *s_ot,
+                                           uint32_t bit, uint32_t strand, uint32_t &l, uint32_t &h, uint32_t &o) {
+    const uint32_t wi = bit >> 5, sh = bit & 31;
+    l = funnel(s_lo[wi + 1], s_lo[wi], sh);
+    h = funnel(s_hi[wi + 1], s_hi[wi], sh);
+    o = funnel(s_ot[wi + 1], s_ot[wi], sh);
+    if (strand) {
+        l = __builtin_bitreverse32(l);
+        o = __builtin_bitreverse32(o);
+        h = ~__builtin_bitreverse32(h) & ~o;  // complement: hi ^= 1 on bases
+    }
+}
+
+template <int NK, int MINEX, int NP>
+__global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
+    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint16_t s_list[kSeedListCap];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ HitStageBuf<kSeedStageCap> s_stage;
+    extern __shared__ __align__(16) uint32_t s_idx[];  // rows, then records at md.recw
+    const MaskDesc &md = a.md;
+    for (uint32_t i = threadIdx.x; i < md.nwords / 4; i += kThreads)
+        reinterpret_cast<uint4 *>(s_idx)[i] = reinterpret_cast<const uint4 *>(a.idx)[i];
+    // visible after tile_candidates' first barrier
+    const uint32_t *s_rows = s_idx;
+    const uint4 *s_rec = reinterpret_cast<const uint4 *>(s_idx + md.recw);
+    const uint32_t care = md.care;
+    const uint32_t fwd_bit = a.cbase + 32 * kHaloL, rev_bit = a.pd.P - 32 - a.cbase + 32 * kHaloL;
+    tile_candidates<1, false, kSeedListCap>(
+        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, stage_ref(s_stage), a.hkey,
+        a.hmm, a.count, a.cap, [&](uint64_t t0, auto, const uint32_t (&e)[1], const bool (&)[1]) {
+            const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
+            uint32_t l, h, o;
+            window_lds(s_lo, s_hi, s_ot, ls + (strand ? rev_bit : fwd_bit), strand, l, h, o);
+            const uint32_t lc = l & care, hc = h & care, oc = o & care;
+            // rows of every range first (independent LDS reads), then combine;
+            // parts past md.parts have span 0 and address the zero row
+            uint32_t m[NP][NK];
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                const uint32_t f = md.first[p];  // bit | span << 8 | first row << 16
+                const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
+                const uint32_t code =
+                    __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
+                const uint32_t pm = ((1u << span) - 1) << a0;
+                load_row<NK>(s_rows, (oc & pm) ? md.zrow : (f >> 16) + code, m[p]);
+            }
+            uint32_t any[NK], two[NK];
+            combine_rows<NP, NK, MINEX == 2>(m, any, two);
+            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(s_stage),
+                                (strand << 15) | ls};
+#pragma unroll
+            for (int w = 0; w < NK; w++) {
+                for (uint32_t x = MINEX == 2 ? two[w] : any[w]; x; x &= x - 1) {
+                    const uint4 k = s_rec[32 * w + __builtin_ctz(x)];
+                    const uint32_t y = __builtin_amdgcn_bitop3_b32(hc, k.y, oc, kXorOr);
+                    const uint32_t mm = (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc, k.x, y, kXorOr));
+                    if ((int32_t)(k.z + mm) < 0) emit_hit(sink, k.w, mm);
                 }
             }
         });
@@ -1260,6 +1416,62 @@ hipError_t launch_seed(const DevGenome &g, const PatternDesc &pd, const SeedDesc
     a.cap = cap;
     const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
     hipLaunchKernelGGL(fused_seed_kernel, grid, dim3(kThreads), (lds_bytes + 15) / 16 * 16, s, a);
+    return hipGetLastError();
+}
+
+template <int NK, int NP>
+static void mask_minex(const MaskArgs &a, dim3 grid, uint32_t lds, hipStream_t s) {
+    if (a.md.minex == 2) hipLaunchKernelGGL((fused_mask_kernel<NK, 2, NP>), grid, dim3(kThreads), lds, s, a);
+    else hipLaunchKernelGGL((fused_mask_kernel<NK, 1, NP>), grid, dim3(kThreads), lds, s, a);
+}
+
+template <int NK>
+static void mask_parts(const MaskArgs &a, dim3 grid, uint32_t lds, hipStream_t s) {
+    if (a.md.parts <= 4) mask_minex<NK, 4>(a, grid, lds, s);
+    else if (a.md.parts <= 6) mask_minex<NK, 6>(a, grid, lds, s);
+    else mask_minex<NK, 8>(a, grid, lds, s);
+}
+
+hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
+                       const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                       unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
+                       uint32_t max_tiles) {
+    const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
+    if (limit == 0 || md.nrec == 0) return hipSuccess;
+    if (tile_stride == 0 || md.parts == 0 || md.parts > (uint32_t)kSeedMaxParts || (md.minex != 1 && md.minex != 2) ||
+        md.nrec > 32 * md.nk || md.nwords % 4 || md.recw % 4 || md.recw + 4 * md.nrec != md.nwords ||
+        (uintptr_t)idx % 16)
+        return hipErrorInvalidValue;
+    for (uint32_t p = 0; p < (uint32_t)kSeedMaxParts; p++) {
+        const uint32_t span = (md.first[p] >> 8) & 0xFF, rbase = md.first[p] >> 16;
+        if (p >= md.parts ? md.first[p] != md.zrow << 16  // padding: span 0 at the zero row
+                          : span == 0 || span > (uint32_t)kMaskMaxSpan || (md.first[p] & 0xFF) + span > 32 ||
+                                rbase + (1u << (2 * span)) > md.zrow)
+            return hipErrorInvalidValue;
+    }
+    if ((uint64_t)(md.zrow + 1) * md.nk > md.recw) return hipErrorInvalidValue;
+    MaskArgs a{};
+    a.g = g;
+    a.pd = pd;
+    a.md = md;
+    a.cbase = cbase;
+    a.tile_stride = tile_stride;
+    a.idx = idx;
+    a.limit = limit;
+    a.hkey = hkey;
+    a.hmm = hmm;
+    a.count = count;
+    a.cand_count = cand_count;
+    a.cap = cap;
+    const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    const uint32_t lds = md.nwords * 4;
+    switch (md.nk) {
+        case 1: mask_parts<1>(a, grid, lds, s); break;
+        case 2: mask_parts<2>(a, grid, lds, s); break;
+        case 4: mask_parts<4>(a, grid, lds, s); break;
+        case 8: mask_parts<8>(a, grid, lds, s); break;
+        default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 

@@ -388,6 +388,75 @@ bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTabl
     return s.lds_bytes <= 24 * 1024;
 }
 
+bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int minex, MaskIndex &s) {
+    s = MaskIndex();
+    const size_t n = keys.size();
+    if (t.gen || t.nw != 1 || t.units != 1 || n == 0 || n > 32u * kMaskMaxWords || (minex != 1 && minex != 2))
+        return false;
+    const uint32_t care = t.words[0];
+    std::vector<int> pos;
+    for (int b = 0; b < 32; b++)
+        if ((care >> b) & 1) pos.push_back(b);
+    const int C = (int)pos.size();
+    MaskDesc &d = s.desc;
+    // ranges: consecutive groups of comparable positions, as equal as
+    // possible, each spanning at most kMaskMaxSpan bits (holes included)
+    int parts = t.tmax + minex;
+    for (;; parts++) {
+        if (parts > kSeedMaxParts || parts > C) return false;
+        bool fits = true;
+        for (int p = 0; p < parts && fits; p++) {
+            const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
+            fits = g1 > g0 && pos[g1 - 1] + 1 - pos[g0] <= kMaskMaxSpan;
+        }
+        if (fits) break;
+    }
+    d.parts = (uint32_t)parts;
+    d.minex = (uint32_t)minex;
+    d.nrec = (uint32_t)n;
+    d.care = care;
+    d.nk = 1;
+    while (32 * d.nk < n) d.nk *= 2;
+    uint32_t rows = 0, rbase[kSeedMaxParts];
+    for (int p = 0; p < parts; p++) {
+        const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
+        const int a = pos[g0], span = pos[g1 - 1] + 1 - a;
+        rbase[p] = rows;
+        d.first[p] = (uint32_t)a | (uint32_t)span << 8 | rows << 16;
+        rows += 1u << (2 * span);
+    }
+    d.zrow = rows++;
+    for (int p = parts; p < kSeedMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
+    d.recw = (rows * d.nk + 3) & ~3u;
+    d.nwords = d.recw + 4 * d.nrec;
+    s.words.assign(d.nwords, 0);
+    for (size_t r = 0; r < n; r++) {
+        const std::string &k = keys[r].key->seq;
+        for (int p = 0; p < parts; p++) {
+            const int a = (int)(d.first[p] & 0xFF), span = (int)((d.first[p] >> 8) & 0xFF);
+            std::vector<uint32_t> cs = {0};
+            for (int i = 0; i < span; i++) {
+                const char c = k[(size_t)(t.cbase + a + i)];
+                std::vector<int> bases;
+                if (c == 'T') bases = {0};
+                else if (c == 'C') bases = {1};
+                else if (c == 'A') bases = {2};
+                else if (c == 'G') bases = {3};
+                else bases = {0, 1, 2, 3};  // N (not comparable): any window base
+                std::vector<uint32_t> nx;
+                for (uint32_t c0 : cs)
+                    for (int b : bases) nx.push_back(c0 | (uint32_t)(b & 1) << i | (uint32_t)(b >> 1) << (span + i));
+                cs.swap(nx);
+            }
+            for (uint32_t c : cs) s.words[(size_t)(rbase[p] + c) * d.nk + r / 32] |= 1u << (r % 32);
+        }
+        // the key table's record r (single class: records follow the header in key order)
+        for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * r + w];
+    }
+    s.lds_bytes = d.nwords * 4;
+    return s.lds_bytes <= 40 * 1024;
+}
+
 namespace {
 
 // Bulge families of every guide of q (see BulgeGuide).  Each compare key is

@@ -111,6 +111,18 @@ struct SeedIndex {
 };
 bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTable &t, SeedIndex &s);
 
+// Bit-set seed index (MaskDesc) of a key table built by build_key_table from
+// `keys`, keeping records that match at least `minex` (1 or 2) ranges
+// exactly.  False when the table does not qualify (IUPAC codes, two words,
+// several care classes, more than 32 * kMaskMaxWords keys, thresholds too
+// high for the comparable positions, or an index over the LDS budget).
+struct MaskIndex {
+    MaskDesc desc{};
+    std::vector<uint32_t> words;
+    uint32_t lds_bytes = 0;
+};
+bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int minex, MaskIndex &s);
+
 // Bulge-family table of queries sharing padded pattern, D, R and PAM side.
 struct BulgeTable {
     std::vector<uint32_t> words;

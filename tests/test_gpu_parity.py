@@ -15,12 +15,13 @@ pytestmark = pytest.mark.gpu
 MODES = (casoff.MODE_PLAIN, casoff.MODE_BATCHED)
 
 
-@pytest.fixture(scope="module", params=["tuned", "0", "1", "2"])
+@pytest.fixture(scope="module", params=["tuned", "0", "1", "2", "3", "4"])
 def ctx(request):
     """A context whose batched launches use the autotuner's variants, or every
-    launch forced to variant 0 / 1 / 2 (fused: 32- / 8-key sign blocks / seed
-    index where the table qualifies; bulge: queued / in-place exact counts), so
-    each kernel variant meets the oracle."""
+    launch forced to variant 0 / 1 / 2 / 3 / 4 (fused: 32- / 8-key sign blocks /
+    seed index / bit-set seed index keeping keys with 1 / 2 exact ranges, the
+    indexes where the table qualifies; bulge: queued / in-place exact counts),
+    so each kernel variant meets the oracle."""
     old = os.environ.pop("COF_FORCE_VARIANT", None)
     if request.param != "tuned":
         os.environ["COF_FORCE_VARIANT"] = request.param
@@ -261,4 +262,34 @@ def test_seed_index_shared_inner_n(ctx, tmp_path, seed):
     lines = [pattern] + ["%s %d" % (g, r.randint(0, tmax)) for g in guides]
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "seed %d" % seed)
+    assert len(exp) > 0
+
+
+@pytest.mark.parametrize("n_guides,tmax", [(30, 2), (60, 4), (100, 4), (100, 6), (250, 3)])
+def test_seed_index_many_keys(ctx, tmp_path, n_guides, tmax):
+    """Single-class NGG tables of 30-250 keys (bit-set rows of 1, 2, 4 and 8
+    words) with planted near-sites and thresholds up to tmax: every index
+    variant must report exactly the oracle's records."""
+    r = random.Random(700 + n_guides + tmax)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(40000, 90000), "ACGTN" if i == 1 else "ACGT", n_runs=2))
+            for i in range(3)]
+    gp = str(tmp_path / "k.fa")
+    H.write_fasta(gp, recs)
+    pattern = "N" * 20 + "NGG"
+    guides = []
+    while len(guides) < n_guides // 2:  # copies of NGG sites with up to tmax substitutions
+        _, sq = r.choice(recs)
+        x = r.randrange(0, len(sq) - 23)
+        if sq[x + 21:x + 23] != "GG" or "N" in sq[x:x + 20]:
+            continue
+        c = list(sq[x:x + 20])
+        for _ in range(r.randint(0, tmax)):
+            c[r.randrange(20)] = r.choice("ACGT")
+        guides.append(("".join(c) + "NNN", tmax))
+    guides += [("".join(r.choice("ACGT") for _ in range(20)) + "NNN", r.randint(0, tmax))
+               for _ in range(n_guides - len(guides))]
+    r.shuffle(guides)
+    lines = [pattern] + ["%s %d" % g for g in guides]
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "many keys %d/%d" % (n_guides, tmax))
     assert len(exp) > 0
