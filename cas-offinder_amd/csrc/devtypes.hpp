@@ -94,8 +94,8 @@ struct SeedDesc {
 // parts - t of the key's ranges exactly; the kernel counts only the records
 // that match at least `minex` ranges (parts >= largest threshold + minex).
 // Layout (words, copied to LDS as is): rows (row i at i * nk; row zrow is all
-// zero: a range holding a genome N at a comparable position matches
-// nothing), then at recw the records [klo, khi, bias, key index].
+// zero: the unused parts up to the kernel's part count address it with span
+// 0), then at recw the records [klo, khi, bias, key index].
 constexpr int kMaskMaxWords = 8;
 constexpr int kMaskMaxSpan = 4;
 struct MaskDesc {

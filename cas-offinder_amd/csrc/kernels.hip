@@ -963,7 +963,11 @@ __global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
             window_lds(s_lo, s_hi, s_ot, ls + (strand ? rev_bit : fwd_bit), strand, l, h, o);
             const uint32_t lc = l & care, hc = h & care, oc = o & care;
             // rows of every range first (independent LDS reads), then combine;
-            // parts past md.parts have span 0 and address the zero row
+            // parts past md.parts have span 0 and address the zero row.  A
+            // genome N reads as base T (lo = hi = 0), so a range holding one
+            // may select keys it does not match: extra survivors that the
+            // exact count below rejects.  No hit is lost: the ranges that a
+            // hit's key matches exactly hold no N.
             uint32_t m[NP][NK];
 #pragma unroll
             for (int p = 0; p < NP; p++) {
@@ -971,8 +975,7 @@ __global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
                 const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
                 const uint32_t code =
                     __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
-                const uint32_t pm = ((1u << span) - 1) << a0;
-                load_row<NK>(s_rows, (oc & pm) ? md.zrow : (f >> 16) + code, m[p]);
+                load_row<NK>(s_rows + (f >> 16) * NK, code, m[p]);
             }
             uint32_t any[NK], two[NK];
             combine_rows<NP, NK, MINEX == 2>(m, any, two);
