@@ -98,6 +98,7 @@ struct SeedDesc {
 // 0), then at recw the records [klo, khi, bias, key index].
 constexpr int kMaskMaxWords = 8;
 constexpr int kMaskMaxSpan = 4;
+constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;
 struct MaskDesc {
     uint32_t parts, nk, minex, nrec;
     uint32_t zrow, recw, nwords, care;  // nwords: rows and records, a multiple of 4

@@ -429,8 +429,9 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
 }  // namespace
 
 hipError_t Lane::init() {
-    hipError_t e = hipStreamCreate(&stream);
+    hipError_t e = prepare_kernels();
     if (e != hipSuccess) return e;
+    if ((e = hipStreamCreate(&stream)) != hipSuccess) return e;
     for (auto &x : ev)
         if ((e = hipEventCreate(&x)) != hipSuccess) return e;
     return hipSuccess;

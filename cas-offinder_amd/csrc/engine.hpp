@@ -88,6 +88,8 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 // place.  tile_stride / max_tiles sample tiles (autotuner trials); 1 / 0 scan
 // the whole genome.
 constexpr int kFusedVariants = 5, kBulgeVariants = 2;
+// One-time kernel attributes (dynamic LDS limits); called when a lane opens.
+hipError_t prepare_kernels();
 hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
                        const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                        unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,

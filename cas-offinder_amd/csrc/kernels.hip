@@ -156,10 +156,10 @@ struct TileRegs {
     uint32_t lo[kStageIters], hi[kStageIters], ot[kStageIters];
 };
 
-__device__ __forceinline__ void load_tile(const DevGenome &g, int64_t w0, TileRegs &r) {
+__device__ __forceinline__ void load_tile(const DevGenome &g, int64_t w0, TileRegs &r, int t) {
 #pragma unroll
     for (int k = 0; k < kStageIters; k++) {
-        const int i = (int)threadIdx.x + k * kThreads;
+        const int i = t + k * kThreads;
         const int64_t w = w0 - kHaloL + i;
         const bool in = i < kLdsWords && w >= 0 && (uint64_t)w < g.nwords;
         r.lo[k] = in ? g.lo[w] : 0u;
@@ -168,10 +168,11 @@ __device__ __forceinline__ void load_tile(const DevGenome &g, int64_t w0, TileRe
     }
 }
 
-__device__ __forceinline__ void store_tile(const TileRegs &r, uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot) {
+__device__ __forceinline__ void store_tile(const TileRegs &r, uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot,
+                                           int t) {
 #pragma unroll
     for (int k = 0; k < kStageIters; k++) {
-        const int i = (int)threadIdx.x + k * kThreads;
+        const int i = t + k * kThreads;
         if (i < kLdsWords) {
             s_lo[i] = r.lo[k];
             s_hi[i] = r.hi[k];
@@ -180,16 +181,18 @@ __device__ __forceinline__ void store_tile(const TileRegs &r, uint32_t *s_lo, ui
     }
 }
 
+// t: the thread's index among the kThreads threads that stage the tile.
 __device__ __forceinline__ void stage_tile(const DevGenome &g, int64_t w0, uint32_t *s_lo, uint32_t *s_hi,
-                                           uint32_t *s_ot) {
+                                           uint32_t *s_ot, int t = (int)threadIdx.x) {
     TileRegs r;
-    load_tile(g, w0, r);
-    store_tile(r, s_lo, s_hi, s_ot);
+    load_tile(g, w0, r, t);
+    store_tile(r, s_lo, s_hi, s_ot, t);
 }
 
 // exclusive scan of `v` over the workgroup (four wave64s); *total gets the sum
-__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total) {
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+__device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total,
+                                               int t = (int)threadIdx.x) {
+    const int lane = t & 63, wave = t >> 6;
     uint32_t x = v;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -271,13 +274,14 @@ __device__ __forceinline__ void emit_hit(const TileSink &o, uint32_t key, uint32
 
 // Writes the staged hits of the pass (all threads, after a barrier) and
 // empties the stage.
+// t: the thread's index among the kThreads threads that share the stage.
 __device__ __forceinline__ void flush_stage(StageRef st, uint64_t t0, uint64_t *hkey, uint16_t *hmm,
-                                            unsigned long long *count, uint64_t cap) {
+                                            unsigned long long *count, uint64_t cap, int t = (int)threadIdx.x) {
     const uint32_t n = min(st.h->n, st.cap);
-    if (threadIdx.x == 0) st.h->base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
+    if (t == 0) st.h->base = n ? atomicAdd(count, (unsigned long long)n) : 0ull;
     __syncthreads();
     const unsigned long long base = st.h->base;
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) {
+    for (uint32_t i = (uint32_t)t; i < n; i += kThreads) {
         const uint64_t e = st.ent[i];
         const unsigned long long o = base + i;
         if (o < cap) {
@@ -286,7 +290,7 @@ __device__ __forceinline__ void flush_stage(StageRef st, uint64_t t0, uint64_t *
         }
     }
     __syncthreads();
-    if (threadIdx.x == 0) st.h->n = 0;
+    if (t == 0) st.h->n = 0;
 }
 
 // NW words of the window planes starting at plane bit `bit` (rd(i) reads word
@@ -633,17 +637,26 @@ __global__ __launch_bounds__(256) void comparer_kernel(ComparerArgs a) {
 // ballots and cross-lane queues.
 // The workgroup's tile is blockIdx.x * tile_stride (a stride above 1 samples
 // tiles spread over the genome: autotuner trials).
-template <int W, bool ALL = false, int LC = kListCap, typename Cands>
+// With G > 1 the workgroup holds G groups of kThreads threads, each with its
+// own tile (tile index blockIdx.x * G + group, below ntile) and LDS arrays
+// (the pointers passed are the group's); the groups run the same number of
+// candidate passes, so that every barrier is reached by all of them.
+// s_nmax: G words of LDS (G > 1 only).
+template <int W, bool ALL = false, int LC = kListCap, int G = 1, typename Cands>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 uint32_t tile_stride, unsigned long long *cand_count,
                                                 uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot, uint16_t *s_list,
                                                 uint32_t *s_wsum, StageRef st, uint64_t *hkey, uint16_t *hmm,
-                                                unsigned long long *count, uint64_t cap, Cands &&cands) {
-    const int t = threadIdx.x;
-    const uint32_t tile = blockIdx.x * tile_stride;
+                                                unsigned long long *count, uint64_t cap, Cands &&cands,
+                                                uint32_t ntile = 0, uint32_t *s_nmax = nullptr) {
+    const int t = G == 1 ? (int)threadIdx.x : (int)(threadIdx.x & (kThreads - 1));
+    const uint32_t grp = G == 1 ? 0u : threadIdx.x / kThreads;
+    const uint32_t gi = G == 1 ? blockIdx.x : blockIdx.x * G + grp;
+    const bool live = G == 1 || gi < ntile;
+    const uint32_t tile = live ? gi * tile_stride : 0u;
     const int64_t w0 = (int64_t)tile * kTileWords;
     const uint64_t ppos = g.tile_ppos[tile];
-    stage_tile(g, w0, s_lo, s_hi, s_ot);
+    stage_tile(g, w0, s_lo, s_hi, s_ot, t);
     if (t == 0) st.h->n = 0;
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;
@@ -659,13 +672,21 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     uint32_t mine = 0;
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
+        if (!live) keep[j] = 0;
         fwd[j] &= keep[j];
         rev[j] &= keep[j];
         mine += __popc(fwd[j]) + __popc(rev[j]);
     }
     uint32_t n;
-    const uint32_t first = block_scan(mine, s_wsum, &n);
-    if (t == 0 && n) atomicAdd(cand_count + (blockIdx.x % kCountSlots) * kSlotStride, (unsigned long long)n);
+    const uint32_t first = block_scan(mine, s_wsum, &n, t);
+    if (t == 0 && n) atomicAdd(cand_count + (gi % kCountSlots) * kSlotStride, (unsigned long long)n);
+    uint32_t nall = n;  // passes of the workgroup: the largest candidate count of its groups
+    if (G > 1) {
+        if (t == 0) s_nmax[grp] = n;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < G; k++) nall = max(nall, s_nmax[k]);
+    }
 
     const int lane = t & 63, wave = t >> 6;
     auto rd = [=](int64_t wi, uint32_t &l, uint32_t &h, uint32_t &o) {
@@ -673,8 +694,8 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
         h = s_hi[wi];
         o = s_ot[wi];
     };
-    for (uint32_t pass = 0; pass < n; pass += LC) {
-        const uint32_t cnt = min(n - pass, (uint32_t)LC);
+    for (uint32_t pass = 0; pass < nall; pass += LC) {
+        const uint32_t cnt = pass < n ? min(n - pass, (uint32_t)LC) : 0u;
         if (first < pass + cnt && first + mine > pass) {  // this thread has entries in the pass
             // entry of bit b of word wi: (wi * 32 + b) << 1 | strand; idx counts
             // from the pass start (it wraps below it: then idx >= cnt)
@@ -708,7 +729,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
             }
         }
         __syncthreads();
-        flush_stage(st, t0, hkey, hmm, count, cap);
+        flush_stage(st, t0, hkey, hmm, count, cap, t);
     }
 }
 
@@ -854,6 +875,7 @@ struct MaskArgs {
     PatternDesc pd;
     MaskDesc md;
     uint32_t cbase, tile_stride;
+    uint32_t ntile;       // tiles of the launch (the workgroups hold G of them)
     const uint32_t *idx;  // rows then records (md.nwords words)
     uint64_t limit;
     uint64_t *hkey;
@@ -940,27 +962,35 @@ __device__ __forceinline__ void window_lds(const uint32_t *s_lo, const uint32_t 
     }
 }
 
-template <int NK, int MINEX, int NP>
-__global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
-    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
-    __shared__ uint16_t s_list[kSeedListCap];
-    __shared__ uint32_t s_wsum[4];
-    __shared__ HitStageBuf<kSeedStageCap> s_stage;
+// G tiles per workgroup (groups of kThreads threads) share one copy of the
+// index in LDS; with G > 1 the candidate list holds kListCap entries.
+template <int NK, int MINEX, int NP, int G>
+__global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a) {  // 6 waves per SIMD
+    constexpr int LC = G == 1 ? kSeedListCap : kListCap;
+    __shared__ uint32_t s_lo[G][kLdsWords], s_hi[G][kLdsWords], s_ot[G][kLdsWords];
+    __shared__ uint16_t s_list[G][LC];
+    __shared__ uint32_t s_wsum[G][4];
+    __shared__ uint32_t s_nmax[G];
+    __shared__ HitStageBuf<kSeedStageCap> s_stage[G];
     extern __shared__ __align__(16) uint32_t s_idx[];  // rows, then records at md.recw
     const MaskDesc &md = a.md;
-    for (uint32_t i = threadIdx.x; i < md.nwords / 4; i += kThreads)
+    for (uint32_t i = threadIdx.x; i < md.nwords / 4; i += kThreads * G)
         reinterpret_cast<uint4 *>(s_idx)[i] = reinterpret_cast<const uint4 *>(a.idx)[i];
     // visible after tile_candidates' first barrier
+    const uint32_t grp = G == 1 ? 0u : threadIdx.x / kThreads;
+    uint32_t *g_lo = s_lo[grp], *g_hi = s_hi[grp], *g_ot = s_ot[grp];
+    HitStageBuf<kSeedStageCap> &g_stage = s_stage[grp];
     const uint32_t *s_rows = s_idx;
     const uint4 *s_rec = reinterpret_cast<const uint4 *>(s_idx + md.recw);
     const uint32_t care = md.care;
     const uint32_t fwd_bit = a.cbase + 32 * kHaloL, rev_bit = a.pd.P - 32 - a.cbase + 32 * kHaloL;
-    tile_candidates<1, false, kSeedListCap>(
-        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, stage_ref(s_stage), a.hkey,
-        a.hmm, a.count, a.cap, [&](uint64_t t0, auto, const uint32_t (&e)[1], const bool (&)[1]) {
+    tile_candidates<1, false, LC, G>(
+        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, g_lo, g_hi, g_ot, s_list[grp], s_wsum[grp],
+        stage_ref(g_stage), a.hkey, a.hmm, a.count, a.cap,
+        [&](uint64_t t0, auto, const uint32_t (&e)[1], const bool (&)[1]) {
             const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
             uint32_t l, h, o;
-            window_lds(s_lo, s_hi, s_ot, ls + (strand ? rev_bit : fwd_bit), strand, l, h, o);
+            window_lds(g_lo, g_hi, g_ot, ls + (strand ? rev_bit : fwd_bit), strand, l, h, o);
             const uint32_t lc = l & care, hc = h & care, oc = o & care;
             // rows of every range first (independent LDS reads), then combine;
             // parts past md.parts have span 0 and address the zero row.  A
@@ -979,7 +1009,7 @@ __global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
             }
             uint32_t any[NK], two[NK];
             combine_rows<NP, NK, MINEX == 2>(m, any, two);
-            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(s_stage),
+            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(g_stage),
                                 (strand << 15) | ls};
 #pragma unroll
             for (int w = 0; w < NK; w++) {
@@ -990,7 +1020,8 @@ __global__ __launch_bounds__(kThreads) void fused_mask_kernel(MaskArgs a) {
                     if ((int32_t)(k.z + mm) < 0) emit_hit(sink, k.w, mm);
                 }
             }
-        });
+        },
+        a.ntile, s_nmax);
 }
 
 // ------------------------------------------------------- device bulge expansion
@@ -1422,17 +1453,64 @@ hipError_t launch_seed(const DevGenome &g, const PatternDesc &pd, const SeedDesc
     return hipGetLastError();
 }
 
+template <int NK, int MINEX, int NP, int G>
+static hipError_t mask_launch(const MaskArgs &a, uint32_t lds, hipStream_t s) {
+    const dim3 grid((a.ntile + G - 1) / G);
+    hipLaunchKernelGGL((fused_mask_kernel<NK, MINEX, NP, G>), grid, dim3(kThreads * G), lds, s, a);
+    return hipGetLastError();
+}
+
+// Dynamic LDS limit of the three-group mask kernels (their static LDS plus an
+// index of up to kMaskMaxIndexBytes exceeds 64 KiB).
+template <int NK, int MINEX>
+static hipError_t mask_attr() {
+    hipError_t e = hipSuccess;
+    const void *k[3] = {(const void *)fused_mask_kernel<NK, MINEX, 4, 3>, (const void *)fused_mask_kernel<NK, MINEX, 6, 3>,
+                        (const void *)fused_mask_kernel<NK, MINEX, 8, 3>};
+    for (const void *f : k)
+        if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
+    return e;
+}
+
+hipError_t prepare_kernels() {
+    hipError_t e = hipSuccess;
+    hipError_t (*fs[])() = {mask_attr<1, 1>, mask_attr<1, 2>, mask_attr<2, 1>, mask_attr<2, 2>,
+                            mask_attr<4, 1>, mask_attr<4, 2>, mask_attr<8, 1>, mask_attr<8, 2>};
+    for (auto f : fs)
+        if (e == hipSuccess) e = f();
+    return e;
+}
+
+// Static LDS of fused_mask_kernel per group: tile planes, candidate list,
+// scan words, stage (and the group's s_nmax word).
+constexpr uint32_t mask_group_lds(int lc) {
+    return 3 * kLdsWords * 4 + lc * 2 + 16 + (uint32_t)sizeof(HitStageBuf<kSeedStageCap>) + 4;
+}
+
+// Groups per workgroup: 3 (shared index, one-pass candidate lists) when that
+// keeps at least as many waves resident per CU as single-tile workgroups.
+static int mask_groups(uint32_t idx_bytes) {
+    constexpr uint32_t kCuLds = 160 * 1024, kMaxWaves = 24;  // waves per CU the kernel's registers allow
+    const uint32_t w1 = std::min(kMaxWaves, kCuLds / (mask_group_lds(kSeedListCap) + idx_bytes) * 4);
+    const uint32_t w3 = std::min(kMaxWaves, kCuLds / (3 * mask_group_lds(kListCap) + idx_bytes) * 12);
+    return w3 >= w1 ? 3 : 1;
+}
+
+template <int NK, int MINEX, int NP>
+static hipError_t mask_groups_launch(const MaskArgs &a, uint32_t lds, hipStream_t s) {
+    return mask_groups(lds) == 3 ? mask_launch<NK, MINEX, NP, 3>(a, lds, s) : mask_launch<NK, MINEX, NP, 1>(a, lds, s);
+}
+
 template <int NK, int NP>
-static void mask_minex(const MaskArgs &a, dim3 grid, uint32_t lds, hipStream_t s) {
-    if (a.md.minex == 2) hipLaunchKernelGGL((fused_mask_kernel<NK, 2, NP>), grid, dim3(kThreads), lds, s, a);
-    else hipLaunchKernelGGL((fused_mask_kernel<NK, 1, NP>), grid, dim3(kThreads), lds, s, a);
+static hipError_t mask_minex(const MaskArgs &a, uint32_t lds, hipStream_t s) {
+    return a.md.minex == 2 ? mask_groups_launch<NK, 2, NP>(a, lds, s) : mask_groups_launch<NK, 1, NP>(a, lds, s);
 }
 
 template <int NK>
-static void mask_parts(const MaskArgs &a, dim3 grid, uint32_t lds, hipStream_t s) {
-    if (a.md.parts <= 4) mask_minex<NK, 4>(a, grid, lds, s);
-    else if (a.md.parts <= 6) mask_minex<NK, 6>(a, grid, lds, s);
-    else mask_minex<NK, 8>(a, grid, lds, s);
+static hipError_t mask_parts(const MaskArgs &a, uint32_t lds, hipStream_t s) {
+    if (a.md.parts <= 4) return mask_minex<NK, 4>(a, lds, s);
+    if (a.md.parts <= 6) return mask_minex<NK, 6>(a, lds, s);
+    return mask_minex<NK, 8>(a, lds, s);
 }
 
 hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
@@ -1452,7 +1530,7 @@ hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc
                                 rbase + (1u << (2 * span)) > md.zrow)
             return hipErrorInvalidValue;
     }
-    if ((uint64_t)(md.zrow + 1) * md.nk > md.recw) return hipErrorInvalidValue;
+    if ((uint64_t)(md.zrow + 1) * md.nk > md.recw || md.nwords * 4 > kMaskMaxIndexBytes) return hipErrorInvalidValue;
     MaskArgs a{};
     a.g = g;
     a.pd = pd;
@@ -1466,16 +1544,15 @@ hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc
     a.count = count;
     a.cand_count = cand_count;
     a.cap = cap;
-    const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    a.ntile = tile_grid(limit, tile_stride, max_tiles).x;
     const uint32_t lds = md.nwords * 4;
     switch (md.nk) {
-        case 1: mask_parts<1>(a, grid, lds, s); break;
-        case 2: mask_parts<2>(a, grid, lds, s); break;
-        case 4: mask_parts<4>(a, grid, lds, s); break;
-        case 8: mask_parts<8>(a, grid, lds, s); break;
+        case 1: return mask_parts<1>(a, lds, s);
+        case 2: return mask_parts<2>(a, lds, s);
+        case 4: return mask_parts<4>(a, lds, s);
+        case 8: return mask_parts<8>(a, lds, s);
         default: return hipErrorInvalidValue;
     }
-    return hipGetLastError();
 }
 
 template <int NW, bool GEN, int NS>

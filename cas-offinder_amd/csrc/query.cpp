@@ -454,7 +454,7 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
         for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * r + w];
     }
     s.lds_bytes = d.nwords * 4;
-    return s.lds_bytes <= 40 * 1024;
+    return s.lds_bytes <= kMaskMaxIndexBytes;
 }
 
 namespace {
