@@ -311,6 +311,35 @@ int build_key_table(const std::vector<KeyRef> &keys, uint32_t P, KeyTable &t, st
     return 0;
 }
 
+namespace {
+// 2-bit codes (lo bits then hi bits, T=0 C=1 A=2 G=3) of key bases
+// k[off, off + span) as a window must hold them to match exactly: a base
+// that is not A/C/G/T (N: not comparable) takes every value.  Writes them
+// to out and returns their number, or -1 if there are more than cap.
+int range_codes(const std::string &k, size_t off, int span, uint32_t *out, int cap) {
+    int n = 1;
+    out[0] = 0;
+    for (int i = 0; i < span; i++) {
+        const char c = k[off + (size_t)i];
+        const int b = c == 'T' ? 0 : c == 'C' ? 1 : c == 'A' ? 2 : c == 'G' ? 3 : -1;
+        const uint32_t lo = 1u << i, hi = 1u << (span + i);
+        if (b >= 0) {
+            const uint32_t add = (b & 1 ? lo : 0) | (b & 2 ? hi : 0);
+            for (int j = 0; j < n; j++) out[j] |= add;
+            continue;
+        }
+        if (4 * n > cap) return -1;
+        for (int j = 0; j < n; j++) {
+            out[n + j] = out[j] | lo;
+            out[2 * n + j] = out[j] | hi;
+            out[3 * n + j] = out[j] | lo | hi;
+        }
+        n *= 4;
+    }
+    return n;
+}
+}  // namespace
+
 bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTable &t, SeedIndex &s) {
     (void)P;
     s = SeedIndex();
@@ -338,44 +367,33 @@ bool build_seed_index(const std::vector<KeyRef> &keys, uint32_t P, const KeyTabl
         obase += (1u << (2 * len)) + 1;
     }
     d.noffs = obase;
-    // (code, record) pairs per range, keys expanded over their N positions
-    std::vector<std::vector<std::pair<uint32_t, uint16_t>>> ent(parts);
-    for (size_t n = 0; n < keys.size(); n++) {
-        const std::string &k = keys[n].key->seq;
-        for (int p = 0; p < parts; p++) {
-            const int a = (int)(d.first[p] & 0xFF), len = (int)(d.first[p] >> 8);
-            std::vector<uint32_t> cs = {0};
-            for (int i = 0; i < len; i++) {
-                const char c = k[(size_t)(t.cbase + a + i)];
-                std::vector<int> bases;
-                if (c == 'T') bases = {0};
-                else if (c == 'C') bases = {1};
-                else if (c == 'A') bases = {2};
-                else if (c == 'G') bases = {3};
-                else bases = {0, 1, 2, 3};  // N (not comparable): any window base
-                std::vector<uint32_t> nx;
-                for (uint32_t c0 : cs)
-                    for (int b : bases) nx.push_back(c0 | (uint32_t)(b & 1) << i | (uint32_t)(b >> 1) << (len + i));
-                cs.swap(nx);
-                if (cs.size() > 64) return false;
-            }
-            for (uint32_t c : cs) ent[p].push_back({c, (uint16_t)n});
-        }
-    }
-    std::vector<uint32_t> offs(obase, 0);
+    // ids per (range, code) by counting sort over the keys' codes (keys
+    // expanded over their N positions), in key order within a code
+    std::vector<uint32_t> offs(obase + 1, 0);
     std::vector<uint16_t> ids;
-    for (int p = 0; p < parts; p++) {
-        std::stable_sort(ent[p].begin(), ent[p].end(),
-                         [](const std::pair<uint32_t, uint16_t> &x, const std::pair<uint32_t, uint16_t> &y) {
-                             return x.first < y.first;
-                         });
-        const uint32_t ncode = 1u << (2 * (d.first[p] >> 8));
-        size_t e = 0;
-        for (uint32_t c = 0; c <= ncode; c++) {
-            offs[d.obase[p] + c] = (uint32_t)ids.size();
-            while (c < ncode && e < ent[p].size() && ent[p][e].first == c) ids.push_back(ent[p][e++].second);
+    uint32_t cs[64];
+    for (int pass = 0; pass < 2; pass++) {
+        for (size_t n = 0; n < keys.size(); n++) {
+            const std::string &k = keys[n].key->seq;
+            for (int p = 0; p < parts; p++) {
+                const int a = (int)(d.first[p] & 0xFF), len = (int)(d.first[p] >> 8);
+                const int nc = range_codes(k, (size_t)(t.cbase + a), len, cs, 64);
+                if (nc < 0) return false;
+                for (int j = 0; j < nc; j++) {
+                    if (pass == 0) offs[d.obase[p] + cs[j] + 1]++;
+                    else ids[offs[d.obase[p] + cs[j]]++] = (uint16_t)n;
+                }
+            }
+        }
+        if (pass == 0) {
+            for (uint32_t i = 0; i < obase; i++) offs[i + 1] += offs[i];
+            ids.resize(offs[obase]);
+        } else {  // offs[i] advanced to the end of list i: shift back to starts
+            for (uint32_t i = obase; i > 0; i--) offs[i] = offs[i - 1];
+            offs[0] = 0;
         }
     }
+    offs.resize(obase);
     if (ids.size() > 0xFFFF) return false;
     d.nids = (uint32_t)ids.size();
     std::vector<uint16_t> all;
@@ -434,21 +452,9 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
         const std::string &k = keys[r].key->seq;
         for (int p = 0; p < parts; p++) {
             const int a = (int)(d.first[p] & 0xFF), span = (int)((d.first[p] >> 8) & 0xFF);
-            std::vector<uint32_t> cs = {0};
-            for (int i = 0; i < span; i++) {
-                const char c = k[(size_t)(t.cbase + a + i)];
-                std::vector<int> bases;
-                if (c == 'T') bases = {0};
-                else if (c == 'C') bases = {1};
-                else if (c == 'A') bases = {2};
-                else if (c == 'G') bases = {3};
-                else bases = {0, 1, 2, 3};  // N (not comparable): any window base
-                std::vector<uint32_t> nx;
-                for (uint32_t c0 : cs)
-                    for (int b : bases) nx.push_back(c0 | (uint32_t)(b & 1) << i | (uint32_t)(b >> 1) << (span + i));
-                cs.swap(nx);
-            }
-            for (uint32_t c : cs) s.words[(size_t)(rbase[p] + c) * d.nk + r / 32] |= 1u << (r % 32);
+            uint32_t cs[1u << (2 * kMaskMaxSpan)];
+            const int nc = range_codes(k, (size_t)(t.cbase + a), span, cs, 1 << (2 * kMaskMaxSpan));
+            for (int j = 0; j < nc; j++) s.words[(size_t)(rbase[p] + cs[j]) * d.nk + r / 32] |= 1u << (r % 32);
         }
         // the key table's record r (single class: records follow the header in key order)
         for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * r + w];
