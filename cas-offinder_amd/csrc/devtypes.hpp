@@ -88,7 +88,8 @@ struct SeedDesc {
 // Bit-set seed index of a single-class concrete key table of at most
 // 32 * kMaskMaxWords keys (fused_mask_kernel).  The comparable positions are
 // split into `parts` bit ranges of at most kMaskMaxSpan bits; per range and
-// 2-bit code (lo bits then hi bits) a row of nk words has bit r set when
+// 2-bit code (lo bits then hi bits) a row of nk words (1, 2 or a multiple of
+// 4; minex 1 only up to 8 words) has bit r set when
 // record r's bases there equal the code (a key with N inside a range is set
 // under every code).  A window within threshold t of a key matches at least
 // parts - t of the key's ranges exactly; the kernel counts only the records
@@ -96,7 +97,7 @@ struct SeedDesc {
 // Layout (words, copied to LDS as is): rows (row i at i * nk; row zrow is all
 // zero: the unused parts up to the kernel's part count address it with span
 // 0), then at recw the records [klo, khi, bias, key index].
-constexpr int kMaskMaxWords = 8;
+constexpr int kMaskMaxWords = 24;
 constexpr int kMaskMaxSpan = 4;
 constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;
 struct MaskDesc {

@@ -885,23 +885,21 @@ struct MaskArgs {
     uint64_t cap;
 };
 
-template <int NK>
-__device__ __forceinline__ void load_row(const uint32_t *s_rows, uint32_t row, uint32_t (&m)[NK]) {
-    if constexpr (NK >= 4) {
-#pragma unroll
-        for (int q = 0; q < NK / 4; q++) {
-            const uint4 v = reinterpret_cast<const uint4 *>(s_rows + row * NK)[q];
-            m[4 * q] = v.x;
-            m[4 * q + 1] = v.y;
-            m[4 * q + 2] = v.z;
-            m[4 * q + 3] = v.w;
-        }
-    } else if constexpr (NK == 2) {
-        const uint2 v = reinterpret_cast<const uint2 *>(s_rows)[row];
+// N consecutive words at p (16-byte aligned for N = 4, 8-byte for N = 2).
+template <int N>
+__device__ __forceinline__ void load_words(const uint32_t *p, uint32_t (&m)[N]) {
+    if constexpr (N == 4) {
+        const uint4 v = *reinterpret_cast<const uint4 *>(p);
+        m[0] = v.x;
+        m[1] = v.y;
+        m[2] = v.z;
+        m[3] = v.w;
+    } else if constexpr (N == 2) {
+        const uint2 v = *reinterpret_cast<const uint2 *>(p);
         m[0] = v.x;
         m[1] = v.y;
     } else {
-        m[0] = s_rows[row];
+        m[0] = p[0];
     }
 }
 
@@ -998,26 +996,45 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
             // may select keys it does not match: extra survivors that the
             // exact count below rejects.  No hit is lost: the ranges that a
             // hit's key matches exactly hold no N.
-            uint32_t m[NP][NK];
+            uint32_t row[NP];
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 const uint32_t f = md.first[p];  // bit | span << 8 | first row << 16
                 const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
                 const uint32_t code =
                     __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
-                load_row<NK>(s_rows + (f >> 16) * NK, code, m[p]);
+                row[p] = ((f >> 16) + code) * NK;  // word offset of the row
             }
-            uint32_t any[NK], two[NK];
-            combine_rows<NP, NK, MINEX == 2>(m, any, two);
             const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(g_stage),
                                 (strand << 15) | ls};
+            auto count = [&](uint32_t r) {  // exact count of surviving record r
+                const uint4 k = s_rec[r];
+                const uint32_t y = __builtin_amdgcn_bitop3_b32(hc, k.y, oc, kXorOr);
+                const uint32_t mm = (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc, k.x, y, kXorOr));
+                if ((int32_t)(k.z + mm) < 0) emit_hit(sink, k.w, mm);
+            };
+            // rows in chunks of CH words (bounded registers for wide rows):
+            // the chunk's words of every range first (independent LDS
+            // reads), then combined, then its survivors counted two words at
+            // a time (fewer loops whose trip count is the wave's largest)
+            constexpr int CH = NK < 4 ? NK : 4;
+#pragma unroll 1
+            for (int c = 0; c < NK / CH; c++) {
+                uint32_t m[NP][CH];
 #pragma unroll
-            for (int w = 0; w < NK; w++) {
-                for (uint32_t x = MINEX == 2 ? two[w] : any[w]; x; x &= x - 1) {
-                    const uint4 k = s_rec[32 * w + __builtin_ctz(x)];
-                    const uint32_t y = __builtin_amdgcn_bitop3_b32(hc, k.y, oc, kXorOr);
-                    const uint32_t mm = (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc, k.x, y, kXorOr));
-                    if ((int32_t)(k.z + mm) < 0) emit_hit(sink, k.w, mm);
+                for (int p = 0; p < NP; p++) load_words<CH>(s_rows + row[p] + CH * c, m[p]);
+                uint32_t any[CH], two[CH];
+                combine_rows<NP, CH, MINEX == 2>(m, any, two);
+                if constexpr (CH == 1) {
+                    for (uint32_t x = MINEX == 2 ? two[0] : any[0]; x; x &= x - 1) count(__builtin_ctz(x));
+                } else {
+#pragma unroll
+                    for (int w = 0; w < CH; w += 2) {
+                        const uint64_t x0 = MINEX == 2 ? (uint64_t)two[w] | (uint64_t)two[w + 1] << 32
+                                                       : (uint64_t)any[w] | (uint64_t)any[w + 1] << 32;
+                        for (uint64_t x = x0; x; x &= x - 1)
+                            count(32 * (CH * c + w) + (uint32_t)__builtin_ctzll(x));
+                    }
                 }
             }
         },
@@ -1474,8 +1491,9 @@ static hipError_t mask_attr() {
 
 hipError_t prepare_kernels() {
     hipError_t e = hipSuccess;
-    hipError_t (*fs[])() = {mask_attr<1, 1>, mask_attr<1, 2>, mask_attr<2, 1>, mask_attr<2, 2>,
-                            mask_attr<4, 1>, mask_attr<4, 2>, mask_attr<8, 1>, mask_attr<8, 2>};
+    hipError_t (*fs[])() = {mask_attr<1, 1>,  mask_attr<1, 2>,  mask_attr<2, 1>,  mask_attr<2, 2>,
+                            mask_attr<4, 1>,  mask_attr<4, 2>,  mask_attr<8, 1>,  mask_attr<8, 2>,
+                            mask_attr<12, 2>, mask_attr<16, 2>, mask_attr<20, 2>, mask_attr<24, 2>};
     for (auto f : fs)
         if (e == hipSuccess) e = f();
     return e;
@@ -1503,7 +1521,9 @@ static hipError_t mask_groups_launch(const MaskArgs &a, uint32_t lds, hipStream_
 
 template <int NK, int NP>
 static hipError_t mask_minex(const MaskArgs &a, uint32_t lds, hipStream_t s) {
-    return a.md.minex == 2 ? mask_groups_launch<NK, 2, NP>(a, lds, s) : mask_groups_launch<NK, 1, NP>(a, lds, s);
+    if (a.md.minex == 2) return mask_groups_launch<NK, 2, NP>(a, lds, s);
+    if constexpr (NK <= 8) return mask_groups_launch<NK, 1, NP>(a, lds, s);  // wider tables: minex 2 only
+    return hipErrorInvalidValue;
 }
 
 template <int NK>
@@ -1551,6 +1571,10 @@ hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc
         case 2: return mask_parts<2>(a, lds, s);
         case 4: return mask_parts<4>(a, lds, s);
         case 8: return mask_parts<8>(a, lds, s);
+        case 12: return mask_parts<12>(a, lds, s);
+        case 16: return mask_parts<16>(a, lds, s);
+        case 20: return mask_parts<20>(a, lds, s);
+        case 24: return mask_parts<24>(a, lds, s);
         default: return hipErrorInvalidValue;
     }
 }

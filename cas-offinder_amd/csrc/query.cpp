@@ -433,8 +433,8 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
     d.minex = (uint32_t)minex;
     d.nrec = (uint32_t)n;
     d.care = care;
-    d.nk = 1;
-    while (32 * d.nk < n) d.nk *= 2;
+    d.nk = n <= 32 ? 1 : n <= 64 ? 2 : (uint32_t)(n + 127) / 128 * 4;  // 1, 2 or a multiple of 4
+    if (minex == 1 && d.nk > 8) return false;
     uint32_t rows = 0, rbase[kSeedMaxParts];
     for (int p = 0; p < parts; p++) {
         const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);

@@ -265,11 +265,12 @@ def test_seed_index_shared_inner_n(ctx, tmp_path, seed):
     assert len(exp) > 0
 
 
-@pytest.mark.parametrize("n_guides,tmax", [(30, 2), (60, 4), (100, 4), (100, 6), (250, 3)])
+@pytest.mark.parametrize("n_guides,tmax", [(30, 2), (60, 4), (100, 4), (100, 6), (250, 3), (420, 5), (700, 6)])
 def test_seed_index_many_keys(ctx, tmp_path, n_guides, tmax):
-    """Single-class NGG tables of 30-250 keys (bit-set rows of 1, 2, 4 and 8
-    words) with planted near-sites and thresholds up to tmax: every index
-    variant must report exactly the oracle's records."""
+    """Single-class NGG tables of 30-700 keys (bit-set rows of 1, 2, 4, 8, 16
+    and 24 words; the wider ones read in 4-word chunks) with planted
+    near-sites and thresholds up to tmax: every index variant must report
+    exactly the oracle's records."""
     r = random.Random(700 + n_guides + tmax)
     recs = [("c%d" % i, H.random_seq(r, r.randint(40000, 90000), "ACGTN" if i == 1 else "ACGT", n_runs=2))
             for i in range(3)]
