@@ -26,16 +26,20 @@ import workloads  # noqa: E402
 METRIC = "guide×Gbp/s bit-exact, batched vs plain on 1 GPU; serving p50/p99 latency"
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
 HBM_PEAK_GBS = 8000.0
-# Comparer roofline unit, counted by the library as n_model_ops:
-#  fused_kernel (queries without bulges): one (candidate, compare key,
-#    32-position word) costs 3 int32 VALU ops in the bit-sliced compare
-#    (2x v_bitop3 + v_bcnt; 6 for keys with IUPAC codes);
-#  bulge_kernel (DNA/RNA bulges expanded on the device): one (candidate,
-#    guide) costs, per 32-position word, 2 ops per window alignment for the
-#    mismatch masks (5 with IUPAC codes), 1 for the no-bulge popcount and 2
-#    (AND + popcount) per bulge family, plus one OR per family.
-# The survey's 4-bit one-hot cost model (0.75 ops per base comparison of every
-# expanded key) is reported beside it.
+# The batched kernels are bound by VALU issue (integer bit work; their HBM
+# reads are 0.375 B per genome position per launch).  roofline.achieved is
+# the dominant kernel's executed int32 lane-ops per launch (rocprofv3
+# SQ_INSTS_VALU x 64, from the committed summary of the same kernel and
+# workload) over its launch time measured here.  Beside it:
+#  - hbm: the algorithmic bytes (the three genome bit planes) per launch;
+#  - comparison_model (counted by the library as n_model_ops): fused_kernel
+#    (queries without bulges) 3 int32 ops per (candidate, compare key,
+#    32-position word) (2x v_bitop3 + v_bcnt; 6 for keys with IUPAC codes);
+#    bulge_kernel per (candidate, guide, word) 2 ops per window alignment (5
+#    with IUPAC codes), 1 for the no-bulge popcount and 2 (AND + popcount) per
+#    bulge family, plus one OR per family.  The seed-index kernels skip most
+#    of that work, so their frac in this unit exceeds 1;
+#  - survey_cost_model: 0.75 ops per base comparison of every expanded key.
 SURVEY_OPS_PER_BASE_CMP = 0.75
 
 
@@ -74,18 +78,25 @@ def kernels_of(variants):
     return "+".join(sorted(names))
 
 
-def measured_traffic(workload):
-    """HBM bytes per fused launch from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (profiles/), or None."""
+def profile_record(workload, kind):
+    """A figure per full-genome launch from a committed rocprofv3 summary of
+    this workload (profiles/*.md, an HTML comment `<!-- kind {json} -->`):
+    kind "traffic" (FETCH_SIZE / WRITE_SIZE bytes) or "valu" (SQ_INSTS_VALU
+    wave-instructions of the dominant kernel)."""
     import glob
     import re
-    for p in glob.glob(os.path.join(ROOT, "profiles", "*.md")):
-        m = re.search(r"<!-- traffic (\{.*\}) -->", open(p).read())
+    for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
+        m = re.search(r"<!-- %s (\{.*\}) -->" % kind, open(p).read())
         if m:
             d = json.loads(m.group(1))
             if d.get("workload") == workload:
-                return d["fetch_bytes"] + d["write_bytes"], os.path.relpath(p, ROOT)
+                return d, os.path.relpath(p, ROOT)
     return None, None
+
+
+def measured_traffic(workload):
+    d, src = profile_record(workload, "traffic")
+    return (d["fetch_bytes"] + d["write_bytes"], src) if d else (None, None)
 
 
 def calibrate_copy(nbytes):
@@ -239,11 +250,26 @@ def main():
     plain_value = n_guides * gbp / s_p if not args.no_plain else float("nan")
     avg_launch_ms = k_b / max(l_b, 1)
     launch_s = avg_launch_ms * 1e-3
-    achieved = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
+    model = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
     survey = SURVEY_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
     hbm_achieved = 0.375 * total / launch_s / 1e9
     traffic, traffic_src = measured_traffic(args.workload)
     kernel = kernels_of(tune["variants"])
+    # VALU: the lane-ops the dominant kernel executes per launch (rocprofv3
+    # SQ_INSTS_VALU x 64 lanes, committed summary of the same kernel and
+    # workload) over the launch time measured here; without such a summary,
+    # the comparison model's ops
+    vrec, vsrc = profile_record(args.workload, "valu")
+    if vrec and vrec.get("kernel") == kernel and l_b == 1:
+        valu_ops = vrec["valu_insts"] * 64
+        work = "%d int32 lane-ops executed per launch (rocprofv3 SQ_INSTS_VALU x 64, %s)" % (valu_ops, vsrc)
+    else:
+        valu_ops, vsrc = st_b["n_model_ops"] / max(l_b, 1), None
+        work = "%d int32 ops (%s)" % (valu_ops, {
+            "fused_kernel": "3 per (candidate, key, 32-position word)",
+            "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
+        }.get(kernel, "comparison-model units of the kernels summed"))
+    achieved = valu_ops / launch_s / 1e12
     out = {
         "metric": METRIC,
         "value": round(value, 2),
@@ -271,20 +297,17 @@ def main():
         "roofline": {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
                      "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
                      "traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": round(avg_launch_ms, 4),
-                     "work_per_launch": "%d int32 ops (%s)" % (st_b["n_model_ops"] / max(l_b, 1), {
-                         "fused_kernel": "3 per (candidate, key, 32-position word)",
-                         "fused_seed_kernel": "3 per (candidate, key, 32-position word) of the brute-force "
-                                              "compare; the seed index counts only the keys that match a range "
-                                              "exactly, so this is brute-force-equivalent work",
-                         "fused_mask_kernel": "3 per (candidate, key, 32-position word) of the brute-force "
-                                              "compare; the bit-set seed index counts only the keys that match "
-                                              "enough ranges exactly, so this is brute-force-equivalent work",
-                         "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
-                     }.get(kernel, "units of the kernels summed")),
-                     "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
-                                           "frac": round(survey / VALU_PEAK_TOPS, 3)},
+                     "work_per_launch": work,
                      "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)}},
+                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)},
+                     "comparison_model": {
+                         "ops_per_launch": int(st_b["n_model_ops"] / max(l_b, 1)), "achieved": round(model, 3),
+                         "frac": round(model / VALU_PEAK_TOPS, 4),
+                         "unit": "3 int32 ops per (candidate, key, 32-position word) for key tables, the bulge "
+                                 "kernel's family model for bulge tables: brute-force-equivalent work, which the "
+                                 "seed indexes skip (frac above 1)"},
+                     "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
+                                           "frac": round(survey / VALU_PEAK_TOPS, 3)}},
     }
     if not args.no_plain and st_p["t_finder_ms"] > 0:
         # plain path's finder (finder_count_kernel + tile-count scan +

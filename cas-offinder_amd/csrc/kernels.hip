@@ -996,14 +996,14 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
             // may select keys it does not match: extra survivors that the
             // exact count below rejects.  No hit is lost: the ranges that a
             // hit's key matches exactly hold no N.
-            uint32_t row[NP];
+            const uint32_t *row[NP];
 #pragma unroll
             for (int p = 0; p < NP; p++) {
                 const uint32_t f = md.first[p];  // bit | span << 8 | first row << 16
                 const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
                 const uint32_t code =
                     __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
-                row[p] = ((f >> 16) + code) * NK;  // word offset of the row
+                row[p] = s_rows + (f >> 16) * NK + code * NK;  // the range's first row is uniform
             }
             const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(g_stage),
                                 (strand << 15) | ls};
@@ -1022,7 +1022,7 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
             for (int c = 0; c < NK / CH; c++) {
                 uint32_t m[NP][CH];
 #pragma unroll
-                for (int p = 0; p < NP; p++) load_words<CH>(s_rows + row[p] + CH * c, m[p]);
+                for (int p = 0; p < NP; p++) load_words<CH>(row[p] + CH * c, m[p]);
                 uint32_t any[CH], two[CH];
                 combine_rows<NP, CH, MINEX == 2>(m, any, two);
                 if constexpr (CH == 1) {
