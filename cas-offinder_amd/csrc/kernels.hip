@@ -189,16 +189,27 @@ __device__ __forceinline__ void stage_tile(const DevGenome &g, int64_t w0, uint3
     store_tile(r, s_lo, s_hi, s_ot, t);
 }
 
+// Inclusive prefix sum over the 64 lanes of a wave with DPP moves: shifts
+// by 1, 2, 4 and 8 lanes within each row of 16 (lanes shifted in from
+// outside the row read 0), then row_bcast:15 adds each row's last lane to
+// the next row and row_bcast:31 adds lane 31 to rows 2 and 3.
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v) {
+    constexpr int kRowShr = 0x110, kRowBcast15 = 0x142, kRowBcast31 = 0x143;
+    uint32_t x = v;
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 1, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 2, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 4, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowShr + 8, 0xF, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast15, 0xA, 0xF, false);
+    x += (uint32_t)__builtin_amdgcn_update_dpp(0, (int)x, kRowBcast31, 0xC, 0xF, false);
+    return x;
+}
+
 // exclusive scan of `v` over the workgroup (four wave64s); *total gets the sum
 __device__ __forceinline__ uint32_t block_scan(uint32_t v, uint32_t *s_wsum, uint32_t *total,
                                                int t = (int)threadIdx.x) {
     const int lane = t & 63, wave = t >> 6;
-    uint32_t x = v;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-        const uint32_t y = __shfl_up(x, d, 64);
-        if (lane >= d) x += y;
-    }
+    const uint32_t x = wave_incl_scan(v);
     if (lane == 63) s_wsum[wave] = x;
     __syncthreads();
     uint32_t off = 0, tot = 0;
@@ -1005,13 +1016,14 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
                     __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
                 row[p] = s_rows + (f >> 16) * NK + code * NK;  // the range's first row is uniform
             }
-            const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(g_stage),
-                                (strand << 15) | ls};
             auto count = [&](uint32_t r) {  // exact count of surviving record r
                 const uint4 k = s_rec[r];
                 const uint32_t y = __builtin_amdgcn_bitop3_b32(hc, k.y, oc, kXorOr);
                 const uint32_t mm = (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc, k.x, y, kXorOr));
-                if ((int32_t)(k.z + mm) < 0) emit_hit(sink, k.w, mm);
+                if ((int32_t)(k.z + mm) < 0)  // the sink is built only on a hit
+                    emit_hit(TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u},
+                                      stage_ref(g_stage), (strand << 15) | ls},
+                             k.w, mm);
             };
             // rows in chunks of CH words (bounded registers for wide rows):
             // the chunk's words of every range first (independent LDS
