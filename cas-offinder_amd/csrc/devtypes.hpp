@@ -93,17 +93,23 @@ struct SeedDesc {
 // record r's bases there equal the code (a key with N inside a range is set
 // under every code).  A window within threshold t of a key matches at least
 // parts - t of the key's ranges exactly; the kernel counts only the records
-// that match at least `minex` ranges (parts >= largest threshold + minex).
+// that match at least `minex` ranges (parts >= largest threshold + minex;
+// minex 3 below: a count per word).
 // Layout (words, copied to LDS as is): rows (row i at i * nk; row zrow is all
 // zero: the unused parts up to the kernel's part count address it with span
 // 0), then at recw the records [klo, khi, bias, key index].
 constexpr int kMaskMaxWords = 24;
 constexpr int kMaskMaxSpan = 4;
 constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;
+// minex 3: thresholds differ across the table; records are ordered by
+// threshold and word w keeps the records that match at least need(w) =
+// parts - (largest threshold in the word) ranges exactly (4-bit fields of
+// `need`, word w at bits 4 * (w % 8) of need[w / 8]).
 struct MaskDesc {
     uint32_t parts, nk, minex, nrec;
     uint32_t zrow, recw, nwords, care;  // nwords: rows and records, a multiple of 4
     uint32_t first[kSeedMaxParts];  // bit of the range's first position | span << 8 | first row << 16
+    uint32_t need[(kMaskMaxWords + 7) / 8];
 };
 
 // Hit sort key: global compare-key index << kHitKeyShift | window start << 1 | strand.

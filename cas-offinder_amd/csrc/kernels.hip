@@ -874,10 +874,11 @@ __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
 // records with a bit-set seed index (MaskDesc).  Per candidate, each range's
 // 2-bit codes select a row of NK words (bit r: record r's bases there equal
 // the window's); the rows of all ranges are combined into the records that
-// match at least MINEX ranges exactly (OR for 1; for 2, bits seen twice), and
-// only those records are counted, each once.  A window within threshold t of
-// a key matches at least parts - t >= MINEX of its ranges, so no hit is
-// missed.  Compared with fused_seed_kernel there are no per-range lists to
+// match at least MINEX ranges exactly (OR for 1; for 2, bits seen twice; for
+// 3, a bit-sliced count per word against the word's need), and only those
+// records are counted, each once.  A window within threshold t of a key
+// matches at least parts - t ranges exactly, at least what its word asks
+// for, so no hit is missed.  Compared with fused_seed_kernel there are no per-range lists to
 // walk: the row loads are independent and the per-lane loop runs over the
 // few surviving records.
 
@@ -953,6 +954,55 @@ __device__ __forceinline__ void combine_rows(const uint32_t (&m)[NP][NK], uint32
                 two[w] = __builtin_amdgcn_bitop3_b32(g2[0], g2[1], g2[2], kOr3) |
                          __builtin_amdgcn_bitop3_b32(g1[0], g1[1], g1[2], kMaj);
         }
+    }
+}
+
+constexpr uint32_t kXor3 = 0x96;  // a ^ b ^ c
+
+// Bits of word w set in at least `need` (uniform) of the NP rows: the
+// bit-sliced count b3 b2 b1 b0 of the set rows from full adders (v_bitop3
+// XOR3 and majority), then the comparison with need.
+template <int NP, int CH>
+__device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w, uint32_t need) {
+    static_assert(NP == 4 || NP == 6 || NP == 8, "row counts 4, 6, 8");
+    auto fa = [](uint32_t x, uint32_t y, uint32_t z, uint32_t &sum, uint32_t &carry) {
+        sum = __builtin_amdgcn_bitop3_b32(x, y, z, kXor3);
+        carry = __builtin_amdgcn_bitop3_b32(x, y, z, kMaj);
+    };
+    uint32_t b0, b1, b2, b3 = 0, s1, c1;
+    fa(m[0][w], m[1][w], m[2][w], s1, c1);
+    if constexpr (NP == 4) {
+        b0 = s1 ^ m[3][w];
+        const uint32_t k = s1 & m[3][w];
+        b1 = c1 ^ k;
+        b2 = c1 & k;
+    } else {
+        uint32_t s2, c2;
+        fa(m[3][w], m[4][w], m[5][w], s2, c2);
+        if constexpr (NP == 6) {
+            b0 = s1 ^ s2;
+            fa(c1, c2, s1 & s2, b1, b2);
+        } else {
+            const uint32_t s3 = m[6][w] ^ m[7][w], c3 = m[6][w] & m[7][w];
+            uint32_t k1, t0, t1;
+            fa(s1, s2, s3, b0, k1);
+            fa(c1, c2, c3, t0, t1);  // weights 2 and 4
+            b1 = t0 ^ k1;
+            const uint32_t k2 = t0 & k1;
+            b2 = t1 ^ k2;
+            b3 = t1 & k2;
+        }
+    }
+    switch (need) {  // uniform: a scalar branch
+        case 1: return b0 | __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
+        case 2: return __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
+        case 3: return __builtin_amdgcn_bitop3_b32(b1, b0, b2 | b3, kAndOr);
+        case 4: return b2 | b3;
+        case 5: return __builtin_amdgcn_bitop3_b32(b2, b1 | b0, b3, kAndOr);
+        case 6: return __builtin_amdgcn_bitop3_b32(b2, b1, b3, kAndOr);
+        case 7: return __builtin_amdgcn_bitop3_b32(b2, b1 & b0, b3, kAndOr);
+        case 8: return b3;
+        default: return 0u;
     }
 }
 
@@ -1035,16 +1085,24 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
                 uint32_t m[NP][CH];
 #pragma unroll
                 for (int p = 0; p < NP; p++) load_words<CH>(row[p] + CH * c, m[p]);
-                uint32_t any[CH], two[CH];
-                combine_rows<NP, CH, MINEX == 2>(m, any, two);
+                uint32_t any[CH], two[CH], sel[CH];
+                if constexpr (MINEX == 3) {  // per-word counts
+#pragma unroll
+                    for (int w = 0; w < CH; w++) {
+                        const uint32_t cw = CH * c + w;
+                        sel[w] = at_least<NP, CH>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
+                    }
+                } else {
+                    combine_rows<NP, CH, MINEX == 2>(m, any, two);
+#pragma unroll
+                    for (int w = 0; w < CH; w++) sel[w] = MINEX == 2 ? two[w] : any[w];
+                }
                 if constexpr (CH == 1) {
-                    for (uint32_t x = MINEX == 2 ? two[0] : any[0]; x; x &= x - 1) count(__builtin_ctz(x));
+                    for (uint32_t x = sel[0]; x; x &= x - 1) count(__builtin_ctz(x));
                 } else {
 #pragma unroll
                     for (int w = 0; w < CH; w += 2) {
-                        const uint64_t x0 = MINEX == 2 ? (uint64_t)two[w] | (uint64_t)two[w + 1] << 32
-                                                       : (uint64_t)any[w] | (uint64_t)any[w + 1] << 32;
-                        for (uint64_t x = x0; x; x &= x - 1)
+                        for (uint64_t x = (uint64_t)sel[w] | (uint64_t)sel[w + 1] << 32; x; x &= x - 1)
                             count(32 * (CH * c + w) + (uint32_t)__builtin_ctzll(x));
                     }
                 }
@@ -1503,9 +1561,11 @@ static hipError_t mask_attr() {
 
 hipError_t prepare_kernels() {
     hipError_t e = hipSuccess;
-    hipError_t (*fs[])() = {mask_attr<1, 1>,  mask_attr<1, 2>,  mask_attr<2, 1>,  mask_attr<2, 2>,
-                            mask_attr<4, 1>,  mask_attr<4, 2>,  mask_attr<8, 1>,  mask_attr<8, 2>,
-                            mask_attr<12, 2>, mask_attr<16, 2>, mask_attr<20, 2>, mask_attr<24, 2>};
+    hipError_t (*fs[])() = {mask_attr<1, 1>,  mask_attr<1, 2>,  mask_attr<1, 3>,  mask_attr<2, 1>,
+                            mask_attr<2, 2>,  mask_attr<2, 3>,  mask_attr<4, 1>,  mask_attr<4, 2>,
+                            mask_attr<4, 3>,  mask_attr<8, 1>,  mask_attr<8, 2>,  mask_attr<8, 3>,
+                            mask_attr<12, 2>, mask_attr<12, 3>, mask_attr<16, 2>, mask_attr<16, 3>,
+                            mask_attr<20, 2>, mask_attr<20, 3>, mask_attr<24, 2>, mask_attr<24, 3>};
     for (auto f : fs)
         if (e == hipSuccess) e = f();
     return e;
@@ -1534,7 +1594,8 @@ static hipError_t mask_groups_launch(const MaskArgs &a, uint32_t lds, hipStream_
 template <int NK, int NP>
 static hipError_t mask_minex(const MaskArgs &a, uint32_t lds, hipStream_t s) {
     if (a.md.minex == 2) return mask_groups_launch<NK, 2, NP>(a, lds, s);
-    if constexpr (NK <= 8) return mask_groups_launch<NK, 1, NP>(a, lds, s);  // wider tables: minex 2 only
+    if (a.md.minex == 3) return mask_groups_launch<NK, 3, NP>(a, lds, s);
+    if constexpr (NK <= 8) return mask_groups_launch<NK, 1, NP>(a, lds, s);  // wider tables: minex 2, 3 only
     return hipErrorInvalidValue;
 }
 
@@ -1551,7 +1612,7 @@ hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc
                        uint32_t max_tiles) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0 || md.nrec == 0) return hipSuccess;
-    if (tile_stride == 0 || md.parts == 0 || md.parts > (uint32_t)kSeedMaxParts || (md.minex != 1 && md.minex != 2) ||
+    if (tile_stride == 0 || md.parts == 0 || md.parts > (uint32_t)kSeedMaxParts || md.minex < 1 || md.minex > 3 ||
         md.nrec > 32 * md.nk || md.nwords % 4 || md.recw % 4 || md.recw + 4 * md.nrec != md.nwords ||
         (uintptr_t)idx % 16)
         return hipErrorInvalidValue;

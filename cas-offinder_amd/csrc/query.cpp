@@ -432,6 +432,18 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
     d.parts = (uint32_t)parts;
     d.minex = (uint32_t)minex;
     d.nrec = (uint32_t)n;
+    // records in key order, or, when minex 2 meets differing thresholds, by
+    // threshold (so that words hold keys of one threshold and each word needs
+    // parts - its largest threshold exact ranges: minex 3)
+    std::vector<size_t> order(n);
+    for (size_t r = 0; r < n; r++) order[r] = r;
+    bool mixed = false;
+    for (size_t r = 1; r < n; r++) mixed = mixed || keys[r].key->threshold != keys[0].key->threshold;
+    if (minex == 2 && mixed) {
+        std::stable_sort(order.begin(), order.end(),
+                         [&](size_t x, size_t y) { return keys[x].key->threshold > keys[y].key->threshold; });
+        d.minex = 3;
+    }
     d.care = care;
     d.nk = n <= 32 ? 1 : n <= 64 ? 2 : (uint32_t)(n + 127) / 128 * 4;  // 1, 2 or a multiple of 4
     if (minex == 1 && d.nk > 8) return false;
@@ -448,16 +460,24 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
     d.recw = (rows * d.nk + 3) & ~3u;
     d.nwords = d.recw + 4 * d.nrec;
     s.words.assign(d.nwords, 0);
+    for (uint32_t w = 0; w < d.nk; w++) {
+        int tw = 0;
+        for (size_t r = 32 * (size_t)w; r < std::min<size_t>(n, 32 * (size_t)w + 32); r++)
+            tw = std::max<int>(tw, keys[order[r]].key->threshold);
+        const uint32_t need = 32 * (size_t)w < n ? (uint32_t)(parts - tw) : 15u;  // an empty word keeps nothing
+        d.need[w / 8] |= need << (4 * (w % 8));
+    }
     for (size_t r = 0; r < n; r++) {
-        const std::string &k = keys[r].key->seq;
+        const size_t kr = order[r];  // the key in record slot r
+        const std::string &k = keys[kr].key->seq;
         for (int p = 0; p < parts; p++) {
             const int a = (int)(d.first[p] & 0xFF), span = (int)((d.first[p] >> 8) & 0xFF);
             uint32_t cs[1u << (2 * kMaskMaxSpan)];
             const int nc = range_codes(k, (size_t)(t.cbase + a), span, cs, 1 << (2 * kMaskMaxSpan));
             for (int j = 0; j < nc; j++) s.words[(size_t)(rbase[p] + cs[j]) * d.nk + r / 32] |= 1u << (r % 32);
         }
-        // the key table's record r (single class: records follow the header in key order)
-        for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * r + w];
+        // the key table's record of key kr (single class: records follow the header in key order)
+        for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * kr + w];
     }
     s.lds_bytes = d.nwords * 4;
     return s.lds_bytes <= kMaskMaxIndexBytes;
