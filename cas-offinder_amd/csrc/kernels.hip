@@ -1023,8 +1023,13 @@ __device__ __forceinline__ void window_lds(const uint32_t *s_lo, const uint32_t 
 
 // G tiles per workgroup (groups of kThreads threads) share one copy of the
 // index in LDS; with G > 1 the candidate list holds kListCap entries.
+// Register budget: six waves per SIMD, or three for wide rows (NK >=
+// kMaskWideWords), whose LDS index allows no more waves anyway; with per-word
+// counts their rows are read and counted for all words first, so that the
+// reads of every chunk issue before the survivor loops.
+constexpr int kMaskWideWords = 12;
 template <int NK, int MINEX, int NP, int G>
-__global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a) {  // 6 waves per SIMD
+__global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fused_mask_kernel(MaskArgs a) {
     constexpr int LC = G == 1 ? kSeedListCap : kListCap;
     __shared__ uint32_t s_lo[G][kLdsWords], s_hi[G][kLdsWords], s_ot[G][kLdsWords];
     __shared__ uint16_t s_list[G][LC];
@@ -1080,6 +1085,32 @@ __global__ __launch_bounds__(kThreads * G, 6) void fused_mask_kernel(MaskArgs a)
             // reads), then combined, then its survivors counted two words at
             // a time (fewer loops whose trip count is the wave's largest)
             constexpr int CH = NK < 4 ? NK : 4;
+            if constexpr (NK >= kMaskWideWords && MINEX == 3) {
+                uint32_t sel[NK];
+#pragma unroll
+                for (int c = 0; c < NK / 4; c++) {
+                    uint32_t m[NP][4];
+#pragma unroll
+                    for (int p = 0; p < NP; p++) load_words<4>(row[p] + 4 * c, m[p]);
+                    if constexpr (MINEX == 3) {
+#pragma unroll
+                        for (int w = 0; w < 4; w++) {
+                            const uint32_t cw = 4 * c + w;
+                            sel[cw] = at_least<NP, 4>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
+                        }
+                    } else {
+                        uint32_t any[4], two[4];
+                        combine_rows<NP, 4, MINEX == 2>(m, any, two);
+#pragma unroll
+                        for (int w = 0; w < 4; w++) sel[4 * c + w] = MINEX == 2 ? two[w] : any[w];
+                    }
+                }
+#pragma unroll
+                for (int w = 0; w < NK; w += 2)
+                    for (uint64_t x = (uint64_t)sel[w] | (uint64_t)sel[w + 1] << 32; x; x &= x - 1)
+                        count(32 * w + (uint32_t)__builtin_ctzll(x));
+                return;
+            }
 #pragma unroll 1
             for (int c = 0; c < NK / CH; c++) {
                 uint32_t m[NP][CH];
