@@ -127,15 +127,10 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     return COF_OK;
 }
 
-// Batched path, one fused launch (one genome pass) per group:
-//  - queries with DNA/RNA bulges that share padded pattern, bulge sizes and
-//    PAM side form a bulge group: bulge_kernel expands their variants on the
-//    device from the guides;
-//  - the other queries with the same padded pattern form a key group whose
-//    compare keys go into one table (fused_kernel).
-// Records carry global key indices.
-int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
-                   const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err) {
+// Tables, indexes and launch signatures of a batched search over qs (see
+// search_batched).
+static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::vector<uint32_t> &gbase,
+                      BatchPlan &P, std::string &err) {
     auto group_by = [&](const std::vector<size_t> &idx, auto keyf) {
         std::vector<std::vector<size_t>> groups;
         std::map<std::string, size_t> gi;
@@ -156,7 +151,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         if (qs[qi]->keys.empty()) continue;
         (qs[qi]->bulge.empty() ? plain : bulged).push_back(qi);
     }
-    std::vector<BulgeTable> btab;
+    std::vector<BulgeTable> &btab = P.btab;
     for (auto &grp : group_by(bulged, [](const Query &q) {
              return q.pattern + "|" + std::to_string(q.D) + "|" + std::to_string(q.R) + (q.rev ? "|r" : "|f");
          })) {
@@ -168,8 +163,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     }
     std::sort(plain.begin(), plain.end());
     const auto kgroups = group_by(plain, [](const Query &q) { return q.pattern; });
-    std::vector<KeyTable> ktab(kgroups.size());
-    std::vector<const Query *> krep(kgroups.size()), brep;
+    std::vector<KeyTable> &ktab = P.ktab;
+    std::vector<const Query *> &krep = P.krep;
+    ktab.assign(kgroups.size(), KeyTable());
+    krep.assign(kgroups.size(), nullptr);
     for (size_t g = 0; g < kgroups.size(); g++) {
         std::vector<KeyRef> refs;
         for (size_t qi : kgroups[g])
@@ -178,10 +175,16 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         krep[g] = qs[kgroups[g][0]].get();
     }
     // seed indexes of the key tables that qualify (an autotuner variant)
-    std::vector<SeedIndex> seeds(ktab.size());
-    std::vector<char> has_seed(ktab.size(), 0);
-    std::vector<MaskIndex> masks[2] = {std::vector<MaskIndex>(ktab.size()), std::vector<MaskIndex>(ktab.size())};
-    std::vector<char> has_mask[2] = {std::vector<char>(ktab.size(), 0), std::vector<char>(ktab.size(), 0)};
+    auto &seeds = P.seeds;
+    auto &has_seed = P.has_seed;
+    auto &masks = P.masks;
+    auto &has_mask = P.has_mask;
+    seeds.assign(ktab.size(), SeedIndex());
+    has_seed.assign(ktab.size(), 0);
+    for (int x = 0; x < 2; x++) {
+        masks[x].assign(ktab.size(), MaskIndex());
+        has_mask[x].assign(ktab.size(), 0);
+    }
     for (size_t g = 0; g < ktab.size(); g++) {
         std::vector<KeyRef> refs;
         for (size_t qi : kgroups[g])
@@ -190,9 +193,13 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         for (int x = 0; x < 2; x++) has_mask[x][g] = build_mask_index(refs, ktab[g], x + 1, masks[x][g]);
     }
     // one device buffer: key tables (each followed by its seed index), then bulge tables
-    std::vector<uint32_t> kw;
-    std::vector<size_t> koff(ktab.size()), soff(ktab.size(), 0), boff(btab.size());
-    std::vector<size_t> moff[2] = {std::vector<size_t>(ktab.size(), 0), std::vector<size_t>(ktab.size(), 0)};
+    auto &kw = P.kw;
+    auto &koff = P.koff, &soff = P.soff, &boff = P.boff;
+    auto &moff = P.moff;
+    koff.assign(ktab.size(), 0);
+    soff.assign(ktab.size(), 0);
+    boff.assign(btab.size(), 0);
+    for (int x = 0; x < 2; x++) moff[x].assign(ktab.size(), 0);
     for (size_t g = 0; g < ktab.size(); g++) {
         koff[g] = kw.size();
         kw.insert(kw.end(), ktab[g].words.begin(), ktab[g].words.end());
@@ -214,6 +221,64 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         kw.insert(kw.end(), btab[g].words.begin(), btab[g].words.end());
         kw.resize((kw.size() + 3) & ~(size_t)3, 0);
     }
+    const size_t nlaunch = ktab.size() + btab.size();
+    auto bucket = [](uint64_t x) {
+        int b = 0;
+        while ((1ull << b) < x) b++;
+        return std::to_string(b);
+    };
+    auto words = [](const void *p, size_t n) {
+        std::string r;
+        const uint32_t *w = static_cast<const uint32_t *>(p);
+        for (size_t k = 0; k < n / 4; k++) r += std::to_string(w[k]) + ",";
+        return r;
+    };
+    std::vector<std::string> &sig = P.sig;
+    sig.assign(nlaunch, std::string());
+    for (size_t i = 0; i < nlaunch; i++) {
+        if (i < ktab.size()) {
+            sig[i] = "fused|" + std::to_string(ktab[i].nw) + (ktab[i].gen ? "g|" : "c|") + krep[i]->pattern + "|" +
+                     bucket(ktab[i].nkeys) + "|" + std::to_string(ktab[i].tmax);
+        } else {
+            const BulgeTable &t = btab[i - ktab.size()];
+            sig[i] = "bulge|" + std::to_string(t.nw) + (t.gen ? "g|" : "c|") + std::to_string(t.ns) +
+                     (t.rev ? "r|" : "f|") + words(t.cb, sizeof t.cb) + "|" + words(t.pat.f, 4 * t.pat.nf) + "|" +
+                     bucket(t.nguides) + "|" + std::to_string(t.tmax);
+        }
+    }
+    return COF_OK;
+}
+
+// Batched path, one fused launch (one genome pass) per group:
+//  - queries with DNA/RNA bulges that share padded pattern, bulge sizes and
+//    PAM side form a bulge group: bulge_kernel expands their variants on the
+//    device from the guides;
+//  - the other queries with the same padded pattern form a key group whose
+//    compare keys go into one table (fused_kernel).
+// Records carry global key indices.
+int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
+                   const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err) {
+    // the plan of the lane's previous search is reused for the same query
+    // objects (it holds them, so their addresses identify them)
+    if (!(L.plan && L.plan->qs == qs && L.plan->gbase == gbase)) {
+        auto P = std::make_shared<BatchPlan>();
+        if (const int rc = build_plan(qs, gbase, *P, err)) return rc;
+        P->qs = qs;
+        P->gbase = gbase;
+        L.plan = std::move(P);
+    }
+    const BatchPlan &P = *L.plan;
+    const auto &ktab = P.ktab;
+    const auto &krep = P.krep;
+    const auto &btab = P.btab;
+    const auto &seeds = P.seeds;
+    const auto &has_seed = P.has_seed;
+    const auto &masks = P.masks;
+    const auto &has_mask = P.has_mask;
+    const auto &kw = P.kw;
+    const auto &koff = P.koff, &soff = P.soff, &boff = P.boff;
+    const auto &moff = P.moff;
+    const auto &sig = P.sig;
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
     // identity of the launch sequence: every kernel argument and the tables
@@ -269,30 +334,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                             t.units, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i), L.d_hkey.n, L.stream, v,
                             stride, max_tiles);
     };
-    auto bucket = [](uint64_t x) {
-        int b = 0;
-        while ((1ull << b) < x) b++;
-        return std::to_string(b);
-    };
-    auto words = [](const void *p, size_t n) {
-        std::string r;
-        const uint32_t *w = static_cast<const uint32_t *>(p);
-        for (size_t k = 0; k < n / 4; k++) r += std::to_string(w[k]) + ",";
-        return r;
-    };
-    std::vector<std::string> sig(nlaunch);
     std::vector<int> variant(nlaunch, 0);
-    for (size_t i = 0; i < nlaunch; i++) {
-        if (i < ktab.size()) {
-            sig[i] = "fused|" + std::to_string(ktab[i].nw) + (ktab[i].gen ? "g|" : "c|") + krep[i]->pattern + "|" +
-                     bucket(ktab[i].nkeys) + "|" + std::to_string(ktab[i].tmax);
-        } else {
-            const BulgeTable &t = btab[i - ktab.size()];
-            sig[i] = "bulge|" + std::to_string(t.nw) + (t.gen ? "g|" : "c|") + std::to_string(t.ns) +
-                     (t.rev ? "r|" : "f|") + words(t.cb, sizeof t.cb) + "|" + words(t.pat.f, 4 * t.pat.nf) + "|" +
-                     bucket(t.nguides) + "|" + std::to_string(t.tmax);
-        }
-    }
     // variants launch i can run (a cached choice for the same signature may
     // name an index the table does not qualify for: then the default runs)
     auto avail = [&](size_t i, int v) {
@@ -442,6 +484,7 @@ void Lane::release() {
     if (graph) (void)hipGraphExecDestroy(graph);
     graph = nullptr;
     graph_key.clear();
+    plan.reset();
     d_hkey.release();
     d_hkey2.release();
     d_hmm.release();

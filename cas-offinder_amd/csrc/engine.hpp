@@ -146,6 +146,25 @@ struct DevBuf {
     }
 };
 
+// Host side of a batched search: the key and bulge tables, seed indexes and
+// launch signatures built from a query set, in one upload buffer.  Kept per
+// lane for the query objects it was built from (it holds them) and reused
+// when the next search names the same ones.
+struct BatchPlan {
+    std::vector<std::shared_ptr<Query>> qs;
+    std::vector<uint32_t> gbase;
+    std::vector<KeyTable> ktab;
+    std::vector<const Query *> krep;
+    std::vector<SeedIndex> seeds;
+    std::vector<char> has_seed;
+    std::vector<MaskIndex> masks[2];
+    std::vector<char> has_mask[2];
+    std::vector<BulgeTable> btab;
+    std::vector<uint32_t> kw;
+    std::vector<size_t> koff, soff, boff, moff[2];
+    std::vector<std::string> sig;
+};
+
 // One in-order stream with its own work buffers; searches on different lanes
 // may run concurrently over the same resident genome.
 struct Lane {
@@ -172,6 +191,7 @@ struct Lane {
     std::vector<int> graph_variants;
     bool graphs = true;
     bool trials = true;  // may run autotuner trials (off where latency matters: serving)
+    std::shared_ptr<BatchPlan> plan;  // of the last batched search
     hipError_t init();
     void release();
 };

@@ -56,3 +56,35 @@ def test_graph_replays_match_eager(tmp_path):
     finally:
         eager.close()
         graph.close()
+
+
+def test_plan_reused_across_searches_and_genomes(tmp_path):
+    """The lane keeps the tables built for the last query set and reuses them
+    when the same query objects are searched again, also after another
+    genome is loaded; a different query set in between rebuilds them.  Every
+    search equals the same queries searched on a fresh context."""
+    ga, gb = str(tmp_path / "a.fa"), str(tmp_path / "b.fa")
+    casoff.synth_write(casoff.synth_spec(seed=31, n_contigs=3, total_len=1_500_000, n_frac=0.01), ga, 0)
+    casoff.synth_write(casoff.synth_spec(seed=32, n_contigs=4, total_len=2_000_000, n_frac=0.01), gb, 0)
+    texts = [t for _, t in workloads.group_inputs(workloads.mixed_guides(5, 50, 3, 6, 0.5, 1), ga)]
+
+    def fresh(genome, sel):
+        c = casoff.Context(0)
+        try:
+            c.load(genome)
+            return c.search([c.query(texts[i]) for i in sel], casoff.MODE_BATCHED).records()
+        finally:
+            c.close()
+
+    ctx = casoff.Context(0)
+    try:
+        qs = [ctx.query(t) for t in texts]
+        sel_all, sel_one = list(range(len(texts))), [len(texts) - 1]
+        for genome in (ga, gb, ga):
+            ctx.load(genome)
+            for sel in (sel_all, sel_one, sel_all, sel_all):
+                got = ctx.search([qs[i] for i in sel], casoff.MODE_BATCHED).records()
+                want = fresh(genome, sel)
+                assert len(got) == len(want) and (got == want).all(), (genome, sel)
+    finally:
+        ctx.close()
