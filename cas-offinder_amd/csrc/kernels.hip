@@ -1271,8 +1271,10 @@ __device__ __forceinline__ uint32_t general_mask(kptr_t kp, const uint32_t (&cls
 // Each wave queues the (candidate, guide) pairs whose family filter passed
 // instead of branching into the exact count for one or two lanes at a time;
 // a full queue (or the end of the candidate chunk) is drained one pair per
-// lane.  An item is the guide record offset and the candidate entry | the
-// mask of passing families << 16.
+// lane.  An item is the guide record offset, the candidate entry | the mask
+// of passing families << 16, and the pair's mismatch masks at the NS
+// alignments (so that the drain needs neither the windows nor the guide's
+// planes).
 constexpr int kBulgeQueue = 64;
 
 // QUEUE = false runs the exact counts in place, for the lanes that passed
@@ -1283,10 +1285,11 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
     __shared__ HitStageBuf<kStageCap> s_stage;
-    __shared__ uint32_t s_qrec[4][kBulgeQueue], s_qinfo[4][kBulgeQueue];
+    __shared__ uint32_t s_qrec[4][QUEUE ? kBulgeQueue : 1], s_qinfo[4][QUEUE ? kBulgeQueue : 1];
+    __shared__ uint32_t s_qm[4][QUEUE ? kBulgeQueue * NS * NW : 1];  // item i's masks at [i * NS * NW, ...)
     constexpr int KP = bulge_planes(NW, GEN), S = bulge_stride(NW, GEN, NS), FB = bulge_family_words(NW);
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-    uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave];
+    uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave], *qm = s_qm[wave];
     const kptr_t tab = (kptr_t)a.table;
     tile_candidates<1, true>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
                              stage_ref(s_stage), a.hkey, a.hmm, a.count, a.cap,
@@ -1300,25 +1303,11 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                 const uint32_t info = qinfo[lane];
                 const kptr_t kp = tab + qrec[lane];
                 const uint32_t ce = info & 0xFFFFu, fam = info >> 16;
-                uint32_t l[NS][NW], h[NS][NW], o[NS][NW], M[NS][NW];
-                bulge_windows<NW, NS>(a, rd, ce, l, h, o);
+                uint32_t M[NS][NW];
 #pragma unroll
-                for (int u = 0; u < NS; u++) {
-                    if (GEN) {
-                        uint32_t cls[5][NW];
-                        class_planes<NW>(l[u], h[u], o[u], cls);
+                for (int u = 0; u < NS; u++)
 #pragma unroll
-                        for (int w = 0; w < NW; w++) M[u][w] = general_mask<NW>(kp, cls, w);
-                    } else {
-#pragma unroll
-                        for (int w = 0; w < NW; w++) {
-                            const uint32_t care = kp[2 * NW + w];
-                            const uint32_t x = __builtin_amdgcn_bitop3_b32(h[u][w] & care, kp[NW + w], o[u][w] & care,
-                                                                           kXorOr);
-                            M[u][w] = __builtin_amdgcn_bitop3_b32(l[u][w] & care, kp[w], x, kXorOr);
-                        }
-                    }
-                }
+                    for (int w = 0; w < NW; w++) M[u][w] = qm[lane * NS * NW + u * NW + w];
                 const kptr_t vb = tab + kp[KP];
                 const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
                                     stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
@@ -1433,6 +1422,10 @@ __global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
                                                            __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
                         qrec[slot] = (uint32_t)(kp - tab);
                         qinfo[slot] = e[0] | (fam << 16);
+#pragma unroll
+                        for (int u = 0; u < NS; u++)
+#pragma unroll
+                            for (int w = 0; w < NW; w++) qm[slot * NS * NW + u * NW + w] = M[u][w];
                     }
                     qn += np;
                 }
