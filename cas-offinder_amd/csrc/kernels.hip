@@ -1207,26 +1207,40 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
                                              const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
                                              const TileSink &sink) {
     constexpr int SW = bulge_seg_words(NW);
+    static_assert(kBulgeSegs == 4, "two halves of two segments");
     const int32_t j0 = (int32_t)fb[0];
     kptr_t ent = tab + fb[2];
-    // all kBulgeSegs records are read (unused ones have no splits), so their
-    // scalar loads issue together
-#pragma unroll
-    for (int sg = 0; sg < kBulgeSegs; sg++) {
-        const kptr_t sr = fb + 4 + sg * SW;
+    // bound of the splits of segments [s0, s1]: left below the first split,
+    // left & right from it to the last split + gap (every position neither
+    // below nor above), right from there on.  An unused segment record is
+    // zero: the bound then counts fewer positions and stays a lower bound.
+    auto bound = [&](int s0, int s1) {
+        const kptr_t r0 = fb + 4 + s0 * SW, r1 = fb + 4 + s1 * SW;
         uint32_t b = bias;
 #pragma unroll
         for (int w = 0; w < NW; w++) {
-            b += (uint32_t)__popc(left[w] & sr[2 + w]);
-            b += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(left[w], right[w], sr[2 + NW + w], kAnd3));
-            b += (uint32_t)__popc(right[w] & sr[2 + 2 * NW + w]);
+            const uint32_t below = r0[2 + w], above = r1[2 + 2 * NW + w];
+            b += (uint32_t)__popc(left[w] & below);
+            b += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(left[w], right[w], ~(below | above), kAnd3));
+            b += (uint32_t)__popc(right[w] & above);
         }
-        const uint32_t a = sr[0], cnt = sr[1];
-        if ((int32_t)b >= 0 || cnt == 0) continue;
-        for (uint32_t k = a; k < a + cnt; k++) {
-            const int32_t j = j0 + (int32_t)k;
-            const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
-            if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+        return (int32_t)b;
+    };
+    // each half (two segments, an unused one has no splits) is bounded
+    // first, its segments only when the half passes
+#pragma unroll
+    for (int hf = 0; hf < 2; hf++) {
+        if (bound(2 * hf, 2 * hf + 1) >= 0) continue;
+#pragma unroll
+        for (int sg = 2 * hf; sg < 2 * hf + 2; sg++) {
+            const kptr_t sr = fb + 4 + sg * SW;
+            const uint32_t a = sr[0], cnt = sr[1];
+            if (cnt == 0 || bound(sg, sg) >= 0) continue;
+            for (uint32_t k = a; k < a + cnt; k++) {
+                const int32_t j = j0 + (int32_t)k;
+                const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
+                if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+            }
         }
     }
 }
