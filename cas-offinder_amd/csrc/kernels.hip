@@ -1585,8 +1585,8 @@ static hipError_t mask_launch(const MaskArgs &a, uint32_t lds, hipStream_t s) {
     return hipGetLastError();
 }
 
-// Dynamic LDS limit of the three-group mask kernels (their static LDS plus an
-// index of up to kMaskMaxIndexBytes exceeds 64 KiB).
+// Dynamic LDS limit of the two- and three-group mask kernels (their static
+// LDS plus an index of up to kMaskMaxIndexBytes exceeds 64 KiB).
 template <int NK, int MINEX>
 static hipError_t mask_attr() {
     hipError_t e = hipSuccess;
@@ -1594,6 +1594,14 @@ static hipError_t mask_attr() {
                         (const void *)fused_mask_kernel<NK, MINEX, 8, 3>};
     for (const void *f : k)
         if (e == hipSuccess) e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
+    if constexpr (NK >= kMaskWideWords) {
+        const void *k2[3] = {(const void *)fused_mask_kernel<NK, MINEX, 4, 2>,
+                             (const void *)fused_mask_kernel<NK, MINEX, 6, 2>,
+                             (const void *)fused_mask_kernel<NK, MINEX, 8, 2>};
+        for (const void *f : k2)
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
+    }
     return e;
 }
 
@@ -1615,18 +1623,32 @@ constexpr uint32_t mask_group_lds(int lc) {
     return 3 * kLdsWords * 4 + lc * 2 + 16 + (uint32_t)sizeof(HitStageBuf<kSeedStageCap>) + 4;
 }
 
-// Groups per workgroup: 3 (shared index, one-pass candidate lists) when that
-// keeps at least as many waves resident per CU as single-tile workgroups.
-static int mask_groups(uint32_t idx_bytes) {
-    constexpr uint32_t kCuLds = 160 * 1024, kMaxWaves = 24;  // waves per CU the kernel's registers allow
-    const uint32_t w1 = std::min(kMaxWaves, kCuLds / (mask_group_lds(kSeedListCap) + idx_bytes) * 4);
-    const uint32_t w3 = std::min(kMaxWaves, kCuLds / (3 * mask_group_lds(kListCap) + idx_bytes) * 12);
-    return w3 >= w1 ? 3 : 1;
+// Groups per workgroup: the count (1, 3, or 2 for wide rows) that keeps the
+// most waves resident per CU under the LDS limit, the larger on a tie (fewer
+// index copies, one-pass candidate lists).
+static int mask_groups(uint32_t idx_bytes, bool wide) {
+    constexpr uint32_t kCuLds = 160 * 1024;
+    const uint32_t max_waves = wide ? 16 : 24;  // the registers of wide-row kernels allow four waves per SIMD
+    auto waves = [&](uint32_t g) {
+        const uint32_t lds = g * mask_group_lds(g == 1 ? kSeedListCap : kListCap) + idx_bytes;
+        return std::min(max_waves, kCuLds / lds * 4 * g);
+    };
+    int best = 1;
+    for (uint32_t g : {3u, 2u})
+        if ((wide || g == 3) && waves(g) >= waves((uint32_t)best)) best = (int)g;
+    return best;
 }
 
 template <int NK, int MINEX, int NP>
 static hipError_t mask_groups_launch(const MaskArgs &a, uint32_t lds, hipStream_t s) {
-    return mask_groups(lds) == 3 ? mask_launch<NK, MINEX, NP, 3>(a, lds, s) : mask_launch<NK, MINEX, NP, 1>(a, lds, s);
+    constexpr bool wide = NK >= kMaskWideWords;
+    switch (mask_groups(lds, wide)) {
+        case 3: return mask_launch<NK, MINEX, NP, 3>(a, lds, s);
+        case 2:
+            if constexpr (wide) return mask_launch<NK, MINEX, NP, 2>(a, lds, s);
+            return hipErrorInvalidValue;
+        default: return mask_launch<NK, MINEX, NP, 1>(a, lds, s);
+    }
 }
 
 template <int NK, int NP>
