@@ -294,3 +294,39 @@ def test_seed_index_many_keys(ctx, tmp_path, n_guides, tmax):
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "many keys %d/%d" % (n_guides, tmax))
     assert len(exp) > 0
+
+
+@pytest.mark.parametrize("n_guides,holes,seed", [(150, 2, 0), (300, 1, 1), (420, 0, 2)])
+def test_seed_index_reversed_pam_holes_mixed_thresholds(ctx, tmp_path, n_guides, holes, seed):
+    """A reversed (5' PAM) pattern, guides sharing inner N positions and
+    thresholds 0-5 mixed in one table: the bit-set index orders records by
+    threshold, expands the holes under every code and, for the wide tables,
+    runs several tile groups per workgroup over a genome of a few tiles."""
+    r = random.Random(900 + seed)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(30000, 70000), "ACGTN" if i == 0 else "ACGT", n_runs=2))
+            for i in range(3)]
+    gp = str(tmp_path / "r.fa")
+    H.write_fasta(gp, recs)
+    pattern = "TTTV" + "N" * 23
+    hole_pos = r.sample(range(3, 20), holes)
+    guides = []
+    while len(guides) < n_guides // 3:  # copies of TTTV sites (forward strand) with substitutions
+        _, sq = r.choice(recs)
+        x = r.randrange(0, len(sq) - 27)
+        w = sq[x:x + 27]
+        if not H.window_passes(pattern, w) or "N" in w[4:]:
+            continue
+        c = list(w[4:])
+        for _ in range(r.randint(0, 3)):
+            c[r.randrange(23)] = r.choice("ACGT")
+        guides.append(c)
+    while len(guides) < n_guides:
+        guides.append([r.choice("ACGT") for _ in range(23)])
+    lines = [pattern]
+    for g in guides:
+        for p in hole_pos:
+            g[p] = "N"
+        lines.append("NNNN%s %d" % ("".join(g), r.randint(0, 5)))
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "reversed holes %d/%d" % (n_guides, holes))
+    assert len(exp) > 0
