@@ -1294,7 +1294,7 @@ constexpr int kBulgeQueue = 64;
 // QUEUE = false runs the exact counts in place, for the lanes that passed
 // (fewer registers; cheaper when filters rarely pass).
 template <int NW, bool GEN, int NS, bool QUEUE>
-__global__ __launch_bounds__(kThreads) void bulge_kernel(BulgeArgs a) {
+__global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
