@@ -218,7 +218,8 @@ def main():
         calibrate_copy(args.calibrate_copy)
         return
     if args.gpus != 1:
-        print("bench.py runs on one GPU only: this build has no multi-GPU path (see DESIGN.md on sharding).",
+        print("bench.py runs on one GPU only: this build has no multi-GPU path (the search would shard by genome "
+              "range or by query, with no exchange step between GPUs).",
               file=sys.stderr)
         sys.exit(2)
 
