@@ -942,10 +942,10 @@ __device__ __forceinline__ void combine_rows(const uint32_t (&m)[NP][NK], uint32
                 g2[g] = 0;
             }
         }
-        if (G == 1) {
+        if constexpr (G == 1) {
             any[w] = g1[0];
             two[w] = g2[0];
-        } else if (G == 2) {
+        } else if constexpr (G == 2) {
             any[w] = g1[0] | g1[1];
             if (TWO) two[w] = __builtin_amdgcn_bitop3_b32(g1[0], g1[1], g2[0] | g2[1], kAndOr);
         } else {
@@ -1054,7 +1054,8 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
         [&](uint64_t t0, auto, const uint32_t (&e)[1], const bool (&)[1]) {
             const uint32_t ls = e[0] >> 1, strand = e[0] & 1;
             uint32_t l, h, o;
-            window_lds(g_lo, g_hi, g_ot, ls + (strand ? rev_bit : fwd_bit), strand, l, h, o);
+            const uint32_t rsel = (uint32_t)((int32_t)(e[0] << 31) >> 31);  // all ones on reverse
+            window_lds(g_lo, g_hi, g_ot, (rsel & (rev_bit - fwd_bit)) + ls + fwd_bit, strand, l, h, o);
             const uint32_t lc = l & care, hc = h & care, oc = o & care;
             // rows of every range first (independent LDS reads), then combine;
             // parts past md.parts have span 0 and address the zero row.  A
@@ -1075,10 +1076,13 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
                 const uint4 k = s_rec[r];
                 const uint32_t y = __builtin_amdgcn_bitop3_b32(hc, k.y, oc, kXorOr);
                 const uint32_t mm = (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(lc, k.x, y, kXorOr));
-                if ((int32_t)(k.z + mm) < 0)  // the sink is built only on a hit
-                    emit_hit(TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u},
-                                      stage_ref(g_stage), (strand << 15) | ls},
+                if ((int32_t)(k.z + mm) < 0) {  // the sink is built only on a hit
+                    uint32_t ev = e[0];
+                    asm volatile("" : "+v"(ev));  // keeps the sink's arithmetic on the (rare) hit path
+                    emit_hit(TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ev >> 1), ev & 1, 0u},
+                                      stage_ref(g_stage), ((ev & 1) << 15) | (ev >> 1)},
                              k.w, mm);
+                }
             };
             // rows in chunks of CH words (bounded registers for wide rows):
             // the chunk's words of every range first (independent LDS
