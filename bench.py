@@ -39,8 +39,10 @@ HBM_PEAK_GBS = 8000.0
 #    with IUPAC codes), 1 for the no-bulge popcount and 2 (AND + popcount) per
 #    bulge family, plus one OR per family.  The seed-index kernels skip most
 #    of that work, so their frac in this unit exceeds 1;
-#  - survey_cost_model: 0.75 ops per base comparison of every expanded key.
-SURVEY_OPS_PER_BASE_CMP = 0.75
+#  - onehot_cost_model: a 4-bit one-hot comparer (8 bases per 32-bit word,
+#    about 6 ops per word), i.e. 0.75 ops per base comparison of every
+#    expanded key with no early exit.
+ONEHOT_OPS_PER_BASE_CMP = 0.75
 
 
 def log(*a):
@@ -252,7 +254,7 @@ def main():
     avg_launch_ms = k_b / max(l_b, 1)
     launch_s = avg_launch_ms * 1e-3
     model = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
-    survey = SURVEY_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
+    onehot = ONEHOT_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
     hbm_achieved = 0.375 * total / launch_s / 1e9
     traffic, traffic_src = measured_traffic(args.workload)
     kernel = kernels_of(tune["variants"])
@@ -307,8 +309,8 @@ def main():
                          "unit": "3 int32 ops per (candidate, key, 32-position word) for key tables, the bulge "
                                  "kernel's family model for bulge tables: brute-force-equivalent work, which the "
                                  "seed indexes skip (frac above 1)"},
-                     "survey_cost_model": {"ops_per_base_cmp": SURVEY_OPS_PER_BASE_CMP, "achieved": round(survey, 2),
-                                           "frac": round(survey / VALU_PEAK_TOPS, 3)}},
+                     "onehot_cost_model": {"ops_per_base_cmp": ONEHOT_OPS_PER_BASE_CMP, "achieved": round(onehot, 2),
+                                           "frac": round(onehot / VALU_PEAK_TOPS, 3)}},
     }
     if not args.no_plain and st_p["t_finder_ms"] > 0:
         # plain path's finder (finder_count_kernel + tile-count scan +
