@@ -61,19 +61,19 @@ constexpr uint32_t match_tt() {
 }
 
 // acc[j] &= class match, for the class set M, of the planes shifted to
-// pattern position k (bit b of word j <-> start b of word i0 + j).
+// pattern position k (bit b of word j <-> start b of word i0 + j).  The match
+// is bitwise, so it is taken on the unshifted words (one v_bitop3 each) and
+// only the one match plane is shifted.
 template <uint32_t M, int NJ, int NL>
 __device__ __forceinline__ void and_match(uint32_t (&acc)[NJ], const uint32_t (&L)[NL], const uint32_t (&H)[NL],
                                           const uint32_t (&O)[NL], uint32_t k) {
+    uint32_t mt[NL];
+#pragma unroll
+    for (int i = 0; i < NL; i++) mt[i] = __builtin_amdgcn_bitop3_b32(L[i], H[i], O[i], match_tt<M>());
     const uint32_t sh = k & 31;
     const bool up = k >= 32;
 #pragma unroll
-    for (int j = 0; j < NJ; j++) {
-        const uint32_t l = funnel(up ? L[j + 2] : L[j + 1], up ? L[j + 1] : L[j], sh);
-        const uint32_t h = funnel(up ? H[j + 2] : H[j + 1], up ? H[j + 1] : H[j], sh);
-        const uint32_t o = funnel(up ? O[j + 2] : O[j + 1], up ? O[j + 1] : O[j], sh);
-        acc[j] &= __builtin_amdgcn_bitop3_b32(l, h, o, match_tt<M>());
-    }
+    for (int j = 0; j < NJ; j++) acc[j] &= funnel(up ? mt[j + 2] : mt[j + 1], up ? mt[j + 1] : mt[j], sh);
 }
 
 // The class set of a pattern position is uniform, so a scalar switch picks
@@ -112,8 +112,21 @@ __device__ __forceinline__ void finder_masks(const PatternDesc &pd, const uint32
     }
 #pragma unroll
     for (int j = 0; j < NJ; j++) fwd[j] = rev[j] = ~0u;
-    for (uint32_t i = 0; i < pd.nf; i++) and_match_any(pd.f[i] >> 8, fwd, L, H, O, pd.f[i] & 0xFF);
-    for (uint32_t i = 0; i < pd.nr; i++) and_match_any(pd.r[i] >> 8, rev, L, H, O, pd.r[i] & 0xFF);
+    // the empty asm marks the planes as changed on every pass, so that the
+    // class matches of the 32 switch cases are not hoisted out of the loops
+    // (all of them live at once would spill)
+    auto opaque = [&]() {
+#pragma unroll
+        for (int i = 0; i < NL; i++) asm volatile("" : "+v"(L[i]), "+v"(H[i]), "+v"(O[i]));
+    };
+    for (uint32_t i = 0; i < pd.nf; i++) {
+        opaque();
+        and_match_any(pd.f[i] >> 8, fwd, L, H, O, pd.f[i] & 0xFF);
+    }
+    for (uint32_t i = 0; i < pd.nr; i++) {
+        opaque();
+        and_match_any(pd.r[i] >> 8, rev, L, H, O, pd.r[i] & 0xFF);
+    }
 }
 
 // Bits of the starts [s0, s0+32) inside [lo, hi).
