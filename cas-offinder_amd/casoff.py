@@ -80,6 +80,8 @@ def lib():
         sig = {
             "cof_version": (C.c_char_p, []),
             "cof_open": (i32, [i32, C.POINTER(vp)]),
+            "cof_device_ids": (i32, [C.c_char_p, i32, C.POINTER(C.c_uint8), C.POINTER(i32)]),
+            "cof_device_count": (i32, [C.POINTER(i32)]),
             "cof_close": (i32, [vp]),
             "cof_last_error": (C.c_char_p, [vp]),
             "cof_genome_load": (i32, [vp, C.c_char_p, C.POINTER(i32)]),
@@ -121,6 +123,18 @@ def _take_buffer(ptr, n):
         return C.string_at(ptr, n).decode("latin-1")
     finally:
         lib().cof_free(ptr)
+
+
+def device_ids(devarg, n_devices):
+    """Device indices (< n_devices) that the CLI's device-id text selects, as
+    Cas_OFFinder(devtype, devarg) reads it (cas-offinder.cpp:131-153), and the
+    number of malformed pieces."""
+    sel = (C.c_uint8 * max(n_devices, 1))()
+    bad = C.c_int(0)
+    rc = lib().cof_device_ids(devarg.encode(), n_devices, sel, C.byref(bad))
+    if rc:
+        raise CofError("cof_device_ids(%r) failed (%d)" % (devarg, rc))
+    return [d for d in range(n_devices) if sel[d]], bad.value
 
 
 def synth_spec(**kw):

@@ -3,6 +3,8 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <string>
+#include <vector>
 
 #include "engine.hpp"
 #include "format.hpp"
@@ -14,6 +16,58 @@ static thread_local std::string g_noctx_err;
 extern "C" {
 
 const char *cof_version(void) { return "3.0.0-mi355x"; }
+
+int cof_device_count(int *n) {
+    if (!n) return COF_E_INVALID;
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *n = std::max(c, 0);
+    return COF_OK;
+}
+
+int cof_device_ids(const char *devarg, int n_devices, uint8_t *selected, int *n_bad) {
+    if (!devarg || n_devices < 0 || (n_devices > 0 && !selected)) return COF_E_INVALID;
+    // getline-style split: no item for an empty text or a trailing delimiter
+    auto split = [](const std::string &t, char d) {
+        std::vector<std::string> v;
+        size_t b = 0;
+        while (b < t.size()) {
+            size_t e = t.find(d, b);
+            if (e == std::string::npos) e = t.size();
+            v.push_back(t.substr(b, e - b));
+            b = e + 1;
+        }
+        return v;
+    };
+    std::vector<uint8_t> sel(n_devices, 0);
+    bool listed = false;  // the reference's dev_ids is non-empty
+    int bad = 0;
+    for (const std::string &piece : split(devarg, ',')) {
+        const std::vector<std::string> ix = split(piece, ':');
+        if (ix.size() == 1) {  // one id; a negative one becomes a huge unsigned index
+            listed = true;
+            const unsigned id = (unsigned)atoi(ix[0].c_str());
+            if (id < (unsigned)n_devices) sel[id] = 1;
+        } else if (ix.size() == 2 || ix.size() == 3) {  // for (j = start; j < end; j += step)
+            const unsigned a = (unsigned)atoi(ix[0].c_str()), b = (unsigned)atoi(ix[1].c_str());
+            const int step = ix.size() == 3 ? atoi(ix[2].c_str()) : 1;
+            if (a >= b) continue;
+            if (step == 0) return COF_E_INVALID;
+            listed = true;
+            for (int d = 0; d < n_devices; d++) {
+                const unsigned u = (unsigned)d;
+                if (step > 0 ? (u >= a && u < b && (u - a) % (unsigned)step == 0)
+                             : (u <= a && (a - u) % (0u - (unsigned)step) == 0))
+                    sel[d] = 1;
+            }
+        } else {
+            bad++;
+        }
+    }
+    for (int d = 0; d < n_devices; d++) selected[d] = listed ? sel[d] : 1;
+    if (n_bad) *n_bad = bad;
+    return COF_OK;
+}
 
 int cof_open(int device, cof_ctx **out) {
     if (!out) return COF_E_INVALID;

@@ -59,12 +59,27 @@ int main(int argc, char **argv) {
     const char dev = pos[1][0];
     if (dev != 'C' && dev != 'G' && dev != 'A') return die("Unknown option: " + pos[1]);
     if (dev != 'G') return die("Only G (the HIP GPU device) is provided by this build.");
-    int device = 0;
-    if (pos[1].size() > 1) device = atoi(pos[1].c_str() + 1);  // first listed id
+    // device ids as the reference reads them (cas-offinder.cpp:131-153, :80-104);
+    // this build searches on the lowest-numbered selected device
+    int ndev = 0, bad = 0;
+    cof_device_count(&ndev);
+    std::vector<uint8_t> sel(ndev > 0 ? ndev : 1, 0);
+    if (cof_device_ids(pos[1].c_str() + 1, ndev, sel.data(), &bad))
+        return die("Something wrong with the device ID argument: " + pos[1]);
+    for (int i = 0; i < bad; i++)
+        std::cerr << "Something wrong with the device ID argument. Use all available devices instead..." << std::endl;
+    int device = -1, nsel = 0;
+    for (int d = 0; d < ndev; d++)
+        if (sel[d]) {
+            if (device < 0) device = d;
+            nsel++;
+        }
+    if (device < 0) return die("No HIP devices found.");
 
     cof_ctx *ctx = nullptr;
     if (cof_open(device, &ctx)) return die(std::string("Device error: ") + cof_last_error(nullptr));
     std::cerr << "Total 1 device(s) found." << std::endl;
+    if (nsel > 1) std::cerr << "Searching on device " << device << " (one device per process)." << std::endl;
     std::cerr << "Loading input file..." << std::endl;
     std::string input;
     if (!slurp(pos[0].c_str(), input)) return die("Critical error! Cannot read input file.");

@@ -97,6 +97,21 @@ int cof_close(cof_ctx *ctx);
 const char *cof_last_error(const cof_ctx *ctx);
 const char *cof_version(void);
 
+/* ---- device ids: the text after the device letter of the CLI's
+ *      {C|G|A}[ids] argument, read as Cas_OFFinder(devtype, devarg) reads it
+ *      (cas-offinder.cpp:131-153) and applied as initOpenCLDevices applies it
+ *      (:80-104): comma-separated ids (atoi) and start:end[:step] ranges; an
+ *      argument that lists no id selects every device.  selected[d] = 1 for
+ *      each device index d < n_devices that it selects, else 0; *n_bad counts
+ *      the pieces that are neither an id nor a range (the reference warns
+ *      "Something wrong with the device ID argument..." and skips them).  A
+ *      negative step counts down from start to 0; a step of 0 over a
+ *      non-empty range is COF_E_INVALID (the reference loops without end).
+ *      Host-only: no device access.  cof_device_count gives the number of
+ *      HIP devices (0 when there is none). */
+int cof_device_ids(const char *devarg, int n_devices, uint8_t *selected, int *n_bad);
+int cof_device_count(int *n);
+
 /* ---- genome: replaces read_fasta (read_fasta.cpp:11-57) / read_twobit
  *      (read_twobit.cpp:44-131) as driven by main.cpp:43-58 and the per-chunk
  *      upload setChrData/loadNextChunk (cas-offinder.cpp:175-252).  `path` is a

@@ -137,7 +137,8 @@ def test_semicolon_and_odd_bytes_in_fasta(ctx, tmp_path):
     check(ctx, tmp_path, gp, text, "odd")
 
 
-def test_cli_end_to_end(ctx, tmp_path):
+@pytest.mark.parametrize("devarg", ["G", "G0:4:2", "G3:1,0"])
+def test_cli_end_to_end(ctx, tmp_path, devarg):
     gp = str(tmp_path / "g.fa")
     casoff.synth_write(casoff.synth_spec(seed=5, n_contigs=3, total_len=200000, n_frac=0.01), gp, 0)
     rr = random.Random(5)
@@ -148,7 +149,7 @@ def test_cli_end_to_end(ctx, tmp_path):
         f.write(text)
     op = str(tmp_path / "out.txt")
     sp = str(tmp_path / "sum.txt")
-    p = subprocess.run([casoff.CLI_PATH, ip, "G", op, "--summary", sp], capture_output=True, timeout=300)
+    p = subprocess.run([casoff.CLI_PATH, ip, devarg, op, "--summary", sp], capture_output=True, timeout=300)
     assert p.returncode == 0, p.stderr
     exp_text, exp_sum = H.oracle_records(tmp_path, text, name="cli_in.txt", summary=True)
     with open(op) as f:
