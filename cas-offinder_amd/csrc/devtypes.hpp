@@ -61,6 +61,10 @@ constexpr int bulge_stride(int nw, bool gen, int ns) { return (bulge_planes(nw, 
 constexpr int bulge_seg_words(int nw) { return 2 + 3 * nw; }
 constexpr int bulge_family_words(int nw) { return 4 + kBulgeSegs * bulge_seg_words(nw); }
 constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that never hits
+// Bulge seed index (bulge_index_kernel): MaskDesc rows, then per slot
+// [glo, ghi, variant block offset, family filter bias of each alignment].
+constexpr int kBulgeSlotWords = 3 + kMaxShifts;
+constexpr uint32_t kBulgeIndexMaxBytes = 32 * 1024;  // static LDS + index stays below 64 KiB
 
 // Statistics counters updated once per workgroup are spread over kCountSlots
 // slots kSlotStride words apart (one 128-byte line each): a single hot address

@@ -216,10 +216,22 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
                 kw.resize((kw.size() + 3) & ~(size_t)3, 0);
             }
     }
+    auto &bidx = P.bidx;
+    auto &has_bidx = P.has_bidx;
+    auto &bioff = P.bioff;
+    bidx.assign(btab.size(), MaskIndex());
+    has_bidx.assign(btab.size(), 0);
+    bioff.assign(btab.size(), 0);
     for (size_t g = 0; g < btab.size(); g++) {
         boff[g] = kw.size();
         kw.insert(kw.end(), btab[g].words.begin(), btab[g].words.end());
         kw.resize((kw.size() + 3) & ~(size_t)3, 0);
+        has_bidx[g] = build_bulge_index(btab[g], bidx[g]);
+        if (has_bidx[g]) {
+            bioff[g] = kw.size();
+            kw.insert(kw.end(), bidx[g].words.begin(), bidx[g].words.end());
+            kw.resize((kw.size() + 3) & ~(size_t)3, 0);
+        }
     }
     const size_t nlaunch = ktab.size() + btab.size();
     auto bucket = [](uint64_t x) {
@@ -278,6 +290,9 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     const auto &kw = P.kw;
     const auto &koff = P.koff, &soff = P.soff, &boff = P.boff;
     const auto &moff = P.moff;
+    const auto &bidx = P.bidx;
+    const auto &has_bidx = P.has_bidx;
+    const auto &bioff = P.bioff;
     const auto &sig = P.sig;
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
@@ -306,6 +321,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(t.gap, sizeof t.gap);
             put(&t.pat, sizeof(PatternDesc));
             put(&boff[g], sizeof boff[g]);
+            put(&bioff[g], sizeof bioff[g]);
         }
         put(kw.data(), kw.size() * 4);
     }
@@ -329,8 +345,13 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             return launch_fused(dg, krep[i]->pat, ktab[i].nw, ktab[i].gen, (uint32_t)ktab[i].cbase,
                                 L.d_keys.p + koff[i], ktab[i].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p,
                                 cand_slots(i), L.d_hkey.n, L.stream, v, stride, max_tiles);
-        const BulgeTable &t = btab[i - ktab.size()];
-        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[i - ktab.size()],
+        const size_t bg = i - ktab.size();
+        const BulgeTable &t = btab[bg];
+        if (v == 2 && has_bidx[bg])
+            return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg], bidx[bg].desc,
+                                      L.d_keys.p + bioff[bg], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
+                                      L.d_hkey.n, L.stream, stride, max_tiles);
+        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg],
                             t.units, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i), L.d_hkey.n, L.stream, v,
                             stride, max_tiles);
     };
@@ -338,7 +359,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // variants launch i can run (a cached choice for the same signature may
     // name an index the table does not qualify for: then the default runs)
     auto avail = [&](size_t i, int v) {
-        if (i >= ktab.size()) return v >= 0 && v < kBulgeVariants;
+        if (i >= ktab.size()) return v == 0 || v == 1 || (v == 2 && has_bidx[i - ktab.size()]);
         switch (v) {
             case 0: case 1: return true;
             case 2: return (bool)has_seed[i];
@@ -351,7 +372,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         variant = L.graph_variants;
     } else if (tuner && tuner->force >= 0) {
         for (size_t i = 0; i < nlaunch; i++)
-            variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : kBulgeVariants - 1;
+            variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : 1;
     } else if (tuner && !(tuner->enabled && L.trials)) {
         for (size_t i = 0; i < nlaunch; i++)  // default when untuned
             if (!tuner->get(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;

@@ -85,9 +85,10 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 // (launch_seed; single-class concrete tables), 3 / 4 = bit-set seed index
 // keeping records that match 1 / 2 ranges exactly (launch_mask; single-class
 // concrete tables of at most 256 keys); bulge 0 = queued exact counts, 1 = in
-// place.  tile_stride / max_tiles sample tiles (autotuner trials); 1 / 0 scan
-// the whole genome.
-constexpr int kFusedVariants = 5, kBulgeVariants = 2;
+// place, 2 = behind the bulge seed index (launch_bulge_index; single-class
+// A/C/G/T tables of at most 128 guides).  tile_stride / max_tiles sample
+// tiles (autotuner trials); 1 / 0 scan the whole genome.
+constexpr int kFusedVariants = 5, kBulgeVariants = 3;
 // One-time kernel attributes (dynamic LDS limits); called when a lane opens.
 hipError_t prepare_kernels();
 hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
@@ -106,6 +107,11 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
                         const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
                         uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
                         hipStream_t s, int variant, uint32_t tile_stride, uint32_t max_tiles);
+hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
+                              const uint32_t *gap, bool rev, const uint32_t *table, const MaskDesc &md,
+                              const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                              unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
+                              uint32_t max_tiles);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
                      uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
 
@@ -160,8 +166,10 @@ struct BatchPlan {
     std::vector<MaskIndex> masks[2];
     std::vector<char> has_mask[2];
     std::vector<BulgeTable> btab;
+    std::vector<MaskIndex> bidx;  // bulge seed indexes (has_bidx)
+    std::vector<char> has_bidx;
     std::vector<uint32_t> kw;
-    std::vector<size_t> koff, soff, boff, moff[2];
+    std::vector<size_t> koff, soff, boff, moff[2], bioff;
     std::vector<std::string> sig;
 };
 

@@ -690,6 +690,102 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
     return 1;
 }
 
+bool build_bulge_index(const BulgeTable &t, MaskIndex &s) {
+    s = MaskIndex();
+    if (t.gen || t.nw != 1 || t.units != 1 || t.words.size() < (size_t)kClassHdr) return false;
+    const int ns = t.ns, KP = bulge_planes(1, false), S = bulge_stride(1, false, ns);
+    const uint32_t care = t.words[0], n = t.words[1];
+    if (n == 0 || n > 128) return false;
+    uint32_t brk = 1;  // ranges one variant can break: max(1, largest RNA gap)
+    for (int u = 0; u < ns; u++) brk = std::max(brk, t.gap[u]);
+    // per guide: the largest threshold among its emitting variants (-1: none)
+    std::vector<int> tg(n, -1);
+    for (uint32_t r = 0; r < n; r++) {
+        const uint32_t *rec = &t.words[kClassHdr + (size_t)r * S];
+        for (int u = 0; u < ns; u++)
+            if (rec[KP + 1 + u] != kNeverHit)
+                tg[r] = std::max(tg[r], -(int32_t)rec[KP + 1 + u] - (int)t.gap[u] - 1);
+    }
+    const int tmax = *std::max_element(tg.begin(), tg.end());
+    if (tmax < 0) return false;
+    std::vector<int> pos;
+    for (int b = 0; b < 32; b++)
+        if ((care >> b) & 1) pos.push_back(b);
+    const int C = (int)pos.size();
+    MaskDesc &d = s.desc;
+    int parts = tmax + (int)brk + 1;  // at least one exact range per surviving guide
+    for (;; parts++) {
+        if (parts > 8 || parts > C) return false;
+        bool fits = true;
+        for (int p = 0; p < parts && fits; p++) {
+            const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
+            fits = g1 > g0 && pos[g1 - 1] + 1 - pos[g0] <= kMaskMaxSpan;
+        }
+        if (fits) break;
+    }
+    // one more range (two exact ones) where it still fits in eight
+    if (parts < 8) {
+        const int p2 = parts + 1;
+        bool fits = p2 <= C;
+        for (int p = 0; p < p2 && fits; p++) {
+            const int g0 = (int)((int64_t)C * p / p2), g1 = (int)((int64_t)C * (p + 1) / p2);
+            fits = g1 > g0 && pos[g1 - 1] + 1 - pos[g0] <= kMaskMaxSpan;
+        }
+        if (fits) parts = p2;
+    }
+    d.parts = (uint32_t)parts;
+    d.minex = 3;
+    d.nrec = n;
+    d.care = care;
+    d.nk = n <= 32 ? 1 : n <= 64 ? 2 : 4;
+    // slots in decreasing threshold order, so that a 32-guide word holds
+    // guides of similar thresholds (its need follows its largest)
+    std::vector<uint32_t> order(n);
+    for (uint32_t r = 0; r < n; r++) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tg[x] > tg[y]; });
+    uint32_t rows = 0, rbase[kSeedMaxParts];
+    for (int p = 0; p < parts; p++) {
+        const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
+        const int a = pos[g0], span = pos[g1 - 1] + 1 - a;
+        rbase[p] = rows;
+        d.first[p] = (uint32_t)a | (uint32_t)span << 8 | rows << 16;
+        rows += 1u << (2 * span);
+    }
+    d.zrow = rows++;
+    for (int p = parts; p < kSeedMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
+    d.recw = (rows * d.nk + 3) & ~3u;
+    // rows, then per slot [glo, ghi, variant block offset, filter bias per alignment]
+    d.nwords = d.recw + kBulgeSlotWords * n;
+    for (uint32_t w = 0; w < d.nk; w++) {
+        int tw = -1;
+        for (uint32_t r = 32 * w; r < std::min<uint32_t>(n, 32 * w + 32); r++) tw = std::max(tw, tg[order[r]]);
+        // a word without an emitting guide keeps nothing
+        const uint32_t need = tw < 0 ? 15u : (uint32_t)(parts - (int)brk - tw);
+        d.need[w / 8] |= need << (4 * (w % 8));
+    }
+    s.words.assign(d.nwords, 0);
+    for (uint32_t r = 0; r < n; r++) {  // slot r holds guide record order[r]
+        const uint32_t *rec = &t.words[kClassHdr + (size_t)order[r] * S];
+        uint32_t *slot = &s.words[d.recw + kBulgeSlotWords * r];
+        slot[0] = rec[0];
+        slot[1] = rec[1];
+        slot[2] = rec[KP];
+        for (int u = 0; u < kMaxShifts; u++) slot[3 + u] = u < ns ? rec[KP + 1 + u] : kNeverHit;
+        const uint32_t glo = rec[0], ghi = rec[1];
+        for (int p = 0; p < parts; p++) {
+            const int a = (int)(d.first[p] & 0xFF), span = (int)((d.first[p] >> 8) & 0xFF);
+            const uint32_t m = ((1u << span) - 1) & (care >> a);  // care bits of the range
+            const uint32_t gl = (glo >> a) & m, gh = (ghi >> a) & m;
+            for (uint32_t code = 0; code < (1u << (2 * span)); code++) {
+                const uint32_t cl = code & ((1u << span) - 1), ch = code >> span;
+                if ((cl & m) == gl && (ch & m) == gh) s.words[(size_t)(rbase[p] + code) * d.nk + r / 32] |= 1u << (r % 32);
+            }
+        }
+    }
+    s.lds_bytes = d.nwords * 4;
+    return s.lds_bytes <= kBulgeIndexMaxBytes;
+}
+
 int encode_query(Query &q, std::string &err) {
     const uint32_t P = q.P;
     if (P == 0) { err = "Empty pattern."; return -1; }

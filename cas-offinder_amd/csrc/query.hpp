@@ -137,6 +137,21 @@ struct BulgeTable {
     uint64_t nguides = 0, nkeys = 0, care_sum = 0;
     int tmax = 0;  // largest threshold of an emitting variant
 };
+// Bit-set seed index of a bulge table (bulge_index_kernel): the guides'
+// comparable positions split into `parts` ranges as in build_mask_index, a
+// row per (range, 2-bit code) of the guides whose bases there equal the code
+// (a position outside the guides' care inside a range matches every code).
+// A variant of split j and RNA gap g compares guide positions below j at one
+// window alignment and positions from j + g at another, so it breaks at most
+// max(1, R) ranges; with threshold t at least parts - max(1, R) - t ranges
+// match exactly at some alignment.  Rows are OR-ed over the alignments and a
+// guide survives when at least `need` of them hold it, need per 32-guide word
+// = parts - max(1, R) - the word's largest threshold (minex 3 layout of
+// MaskDesc, no records: the bulge table's own records follow).  False for
+// IUPAC guides, several care classes, two-word windows, more than 128 guides
+// or thresholds too high for the comparable positions.
+bool build_bulge_index(const BulgeTable &t, MaskIndex &s);
+
 struct BulgeRef {
     const Query *q;
     uint32_t gbase;  // global index of the query's first key

@@ -20,7 +20,8 @@ def ctx(request):
     """A context whose batched launches use the autotuner's variants, or every
     launch forced to variant 0 / 1 / 2 / 3 / 4 (fused: 32- / 8-key sign blocks /
     seed index / bit-set seed index keeping keys with 1 / 2 exact ranges, the
-    indexes where the table qualifies; bulge: queued / in-place exact counts),
+    indexes where the table qualifies; bulge: queued / in-place exact counts /
+    bulge seed index where the table qualifies, in-place counts otherwise),
     so each kernel variant meets the oracle."""
     old = os.environ.pop("COF_FORCE_VARIANT", None)
     if request.param != "tuned":
@@ -238,6 +239,32 @@ def test_bulge_families(ctx, tmp_path, case):
     lines = ["%s %d %d" % (pattern, D, R)] + ["%s %d g%d" % (g, r.randint(1, 5), i) for i, g in enumerate(guides)]
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "bulge case %d" % case)
+    assert len(exp) > 0
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_bulge_index_holes_and_many_guides(ctx, tmp_path, seed):
+    """Bulge groups of one care class with N at shared inner guide positions
+    (ranges whose hole matches every code) and 40-120 guides (index rows of
+    two and four words), thresholds 0-4, DNA and RNA bulges."""
+    r = random.Random(700 + seed)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(30000, 70000), "ACGT", n_runs=r.randint(0, 2))) for i in range(3)]
+    gp = str(tmp_path / "bh.fa")
+    H.write_fasta(gp, recs)
+    pattern = "N" * 21 + "RG"
+    D, R = [(1, 1), (2, 1), (1, 2)][seed]
+    n = [40, 75, 120][seed]
+    holes = r.sample(range(2, 18), seed)  # none, one or two shared inner N
+    cores = []
+    while len(cores) < n:
+        for c in _planted_guides(r, recs, pattern, 20, min(8, n - len(cores))):
+            g = list(c)
+            for p in holes:
+                g[p] = "N"
+            cores.append("".join(g))
+    lines = ["%s %d %d" % (pattern, D, R)] + ["%sNNN %d g%d" % (g, r.randint(0, 4), i) for i, g in enumerate(cores)]
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "bulge index seed %d" % seed)
     assert len(exp) > 0
 
 
