@@ -723,15 +723,19 @@ bool build_bulge_index(const BulgeTable &t, MaskIndex &s) {
         }
         if (fits) break;
     }
-    // one more range (two exact ones) where it still fits in eight
-    if (parts < 8) {
-        const int p2 = parts + 1;
-        bool fits = p2 <= C;
+    // as many ranges as fit, up to eight: each costs a row per alignment, but
+    // every extra range raises the need by one and the survivors (whose
+    // loop is the wave's longest) drop faster than the row reads grow
+    for (int p2 = std::min(8, C); p2 > parts; p2--) {
+        bool fits = true;
         for (int p = 0; p < p2 && fits; p++) {
             const int g0 = (int)((int64_t)C * p / p2), g1 = (int)((int64_t)C * (p + 1) / p2);
             fits = g1 > g0 && pos[g1 - 1] + 1 - pos[g0] <= kMaskMaxSpan;
         }
-        if (fits) parts = p2;
+        if (fits) {
+            parts = p2;
+            break;
+        }
     }
     d.parts = (uint32_t)parts;
     d.minex = 3;
