@@ -69,12 +69,12 @@ def build_workload(name, ctx, n_guides=None):
 def kernels_of(variants):
     """Kernels the batched path launched, from the autotuner's decisions
     (signature -> variant): fused_kernel, fused_seed_kernel (variant 2) or
-    fused_mask_kernel (variants 3, 4) for key groups, bulge_kernel for groups
-    with DNA/RNA bulges."""
+    fused_mask_kernel (variants 3, 4) for key groups, bulge_kernel (variants
+    0, 1) or bulge_index_kernel (variant 2) for groups with DNA/RNA bulges."""
     names = set()
     for sig, v in variants.items():
         if sig.startswith("bulge|"):
-            names.add("bulge_kernel")
+            names.add("bulge_index_kernel" if v == 2 else "bulge_kernel")
         else:
             names.add({2: "fused_seed_kernel", 3: "fused_mask_kernel", 4: "fused_mask_kernel"}.get(v, "fused_kernel"))
     return "+".join(sorted(names))
@@ -271,6 +271,7 @@ def main():
         work = "%d int32 ops (%s)" % (valu_ops, {
             "fused_kernel": "3 per (candidate, key, 32-position word)",
             "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
+            "bulge_index_kernel": "bulge_kernel's family model, brute-force-equivalent",
         }.get(kernel, "comparison-model units of the kernels summed"))
     achieved = valu_ops / launch_s / 1e12
     out = {

@@ -10,7 +10,7 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 WORKLOADS = {"genome_3g_ngg", "bulge_3g_nrg", "mixed_pam_2000"}
-KERNELS = {"fused_kernel", "fused_seed_kernel", "fused_mask_kernel", "bulge_kernel"}
+KERNELS = {"fused_kernel", "fused_seed_kernel", "fused_mask_kernel", "bulge_kernel", "bulge_index_kernel"}
 
 
 def _bench():
