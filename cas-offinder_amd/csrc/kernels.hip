@@ -1471,11 +1471,11 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
 // of split j and RNA gap g compares the guide positions below j at one window
 // alignment and those from j + g at another, so it breaks at most max(1, R)
 // of the index's ranges and, within threshold t, matches at least
-// parts - max(1, R) - t of them exactly at one of the alignments.  Per
-// candidate each range's row is read at every alignment and the rows are
-// OR-ed; a guide survives when at least its word's need of the OR-ed rows
-// hold it, and only survivors get the family filters and exact counts of
-// bulge_kernel (in place).  Index (LDS): rows of nk words per (range, code),
+// parts - max(1, R) - t of them exactly at alignment 0 or at the family's
+// alignment u.  Per candidate each range's row is read at every alignment;
+// per family u the rows at 0 and u are OR-ed and a guide survives when, for
+// some family, at least its word's need of them hold it.  Only survivors get
+// the family filters and exact counts of bulge_kernel (in place).  Index (LDS): rows of nk words per (range, code),
 // then per slot [glo, ghi, variant block offset, family filter bias per
 // alignment] (kBulgeSlotWords words).
 struct BulgeIndexArgs {
@@ -1485,7 +1485,7 @@ struct BulgeIndexArgs {
 };
 
 template <int NK, int NS, int NP>
-__global__ __launch_bounds__(kThreads, NK == 1 ? 5 : NK == 2 ? 4 : 3) void bulge_index_kernel(BulgeIndexArgs ia) {
+__global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(BulgeIndexArgs ia) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];
@@ -1505,22 +1505,32 @@ __global__ __launch_bounds__(kThreads, NK == 1 ? 5 : NK == 2 ? 4 : 3) void bulge
                               [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&)[1]) {
         uint32_t l[NS][1], h[NS][1], o[NS][1];
         bulge_windows<1, NS>(a, rd, e[0], l, h, o);
-        // every range's rows at every alignment (independent LDS reads), OR-ed
-        uint32_t m[NP][NK];
-#pragma unroll
-        for (int p = 0; p < NP; p++) {
+        // row of range p at alignment u
+        auto row = [&](int p, int u, uint32_t (&r)[NK]) {
             const uint32_t f = md.first[p];  // bit | span << 8 | first row << 16
             const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
-            const uint32_t *rb = s_idx + (f >> 16) * NK;  // padding parts: span 0, the zero row
+            const uint32_t code =
+                __builtin_amdgcn_ubfe(l[u][0], a0, span) | (__builtin_amdgcn_ubfe(h[u][0], a0, span) << span);
+            load_words<NK>(s_idx + (f >> 16) * NK + code * NK, r);  // padding parts: span 0, the zero row
+        };
+        // a variant of family u compares every unbroken range at alignment 0
+        // or u: per family, the ranges exact at either are counted
+        uint32_t m0[NP][NK], sel[NK];
 #pragma unroll
-            for (int u = 0; u < NS; u++) {
-                const uint32_t code =
-                    __builtin_amdgcn_ubfe(l[u][0], a0, span) | (__builtin_amdgcn_ubfe(h[u][0], a0, span) << span);
-                uint32_t r[NK];
-                load_words<NK>(rb + code * NK, r);
+        for (int p = 0; p < NP; p++) row(p, 0, m0[p]);
 #pragma unroll
-                for (int k = 0; k < NK; k++) m[p][k] = u == 0 ? r[k] : m[p][k] | r[k];
+        for (int w = 0; w < NK; w++) sel[w] = 0;
+#pragma unroll
+        for (int u = 1; u < NS; u++) {
+            uint32_t m[NP][NK];
+#pragma unroll
+            for (int p = 0; p < NP; p++) {
+                row(p, u, m[p]);
+#pragma unroll
+                for (int k = 0; k < NK; k++) m[p][k] |= m0[p][k];
             }
+#pragma unroll
+            for (int w = 0; w < NK; w++) sel[w] |= at_least<NP, NK>(m, w, (md.need[w / 8] >> (4 * (w % 8))) & 15u);
         }
         uint32_t lc[NS], hc[NS], oc[NS];
 #pragma unroll
@@ -1532,9 +1542,8 @@ __global__ __launch_bounds__(kThreads, NK == 1 ? 5 : NK == 2 ? 4 : 3) void bulge
         const uint32_t ce = e[0];
 #pragma unroll
         for (int w = 0; w < NK; w++) {
-            for (uint32_t sel = at_least<NP, NK>(m, w, (md.need[w / 8] >> (4 * (w % 8))) & 15u); sel;
-                 sel &= sel - 1) {
-                const uint32_t slot = 32 * w + (uint32_t)__builtin_ctz(sel);
+            for (uint32_t x = sel[w]; x; x &= x - 1) {
+                const uint32_t slot = 32 * w + (uint32_t)__builtin_ctz(x);
                 const uint4 s0 = s_slot[2 * slot], s1 = s_slot[2 * slot + 1];
                 const uint32_t bias[5] = {s0.w, s1.x, s1.y, s1.z, s1.w};
                 uint32_t M[NS][1];
