@@ -170,6 +170,20 @@ struct TileRegs {
 };
 
 __device__ __forceinline__ void load_tile(const DevGenome &g, int64_t w0, TileRegs &r, int t) {
+    if (w0 >= kHaloL && (uint64_t)(w0 - kHaloL + kLdsWords) <= g.nwords) {
+        // whole tile and halo inside the planes (every tile but the first
+        // and the last): one base per plane, 32-bit offsets
+        const uint32_t *lo = g.lo + (w0 - kHaloL), *hi = g.hi + (w0 - kHaloL), *ot = g.ot + (w0 - kHaloL);
+#pragma unroll
+        for (int k = 0; k < kStageIters; k++) {
+            const int i = t + k * kThreads;
+            const bool in = k + 1 < kStageIters || i < kLdsWords;
+            r.lo[k] = in ? lo[i] : 0u;
+            r.hi[k] = in ? hi[i] : 0u;
+            r.ot[k] = in ? ot[i] : ~0u;
+        }
+        return;
+    }
 #pragma unroll
     for (int k = 0; k < kStageIters; k++) {
         const int i = t + k * kThreads;
@@ -687,10 +701,11 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     uint32_t fwd[kWordsPerThread], rev[kWordsPerThread], keep[kWordsPerThread];
     uint64_t s0[kWordsPerThread];
     finder_masks(pd, s_lo, s_hi, s_ot, kWordsPerThread * t + kHaloL, fwd, rev);  // words 4t .. 4t+3
+    const bool whole = t1 <= limit;  // every start of the tile is valid (all tiles but the last)
 #pragma unroll
     for (int j = 0; j < kWordsPerThread; j++) {
         s0[j] = t0 + 32ull * (kWordsPerThread * t + j);
-        keep[j] = range_mask(s0[j], 0, limit);
+        keep[j] = whole ? ~0u : range_mask(s0[j], 0, limit);
     }
     apply_poison(g, pd.P, tile, t1, ppos, s0, keep);
     uint32_t mine = 0;
