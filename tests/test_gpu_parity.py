@@ -242,19 +242,20 @@ def test_bulge_families(ctx, tmp_path, case):
     assert len(exp) > 0
 
 
-@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("seed", range(5))
 def test_bulge_index_holes_and_many_guides(ctx, tmp_path, seed):
     """Bulge groups of one care class with N at shared inner guide positions
-    (ranges whose hole matches every code) and 40-120 guides (index rows of
-    two and four words), thresholds 0-4, DNA and RNA bulges."""
+    (ranges whose hole matches every code) and 33-120 guides (index rows of
+    two and four words, full and one-guide words), thresholds 0-4, DNA and
+    RNA bulges."""
     r = random.Random(700 + seed)
     recs = [("c%d" % i, H.random_seq(r, r.randint(30000, 70000), "ACGT", n_runs=r.randint(0, 2))) for i in range(3)]
     gp = str(tmp_path / "bh.fa")
     H.write_fasta(gp, recs)
     pattern = "N" * 21 + "RG"
-    D, R = [(1, 1), (2, 1), (1, 2)][seed]
-    n = [40, 75, 120][seed]
-    holes = r.sample(range(2, 18), seed)  # none, one or two shared inner N
+    D, R = [(1, 1), (2, 1), (1, 2), (0, 1), (2, 0)][seed]
+    n = [40, 75, 120, 33, 64][seed]
+    holes = r.sample(range(2, 18), seed % 3)  # none, one or two shared inner N
     cores = []
     while len(cores) < n:
         for c in _planted_guides(r, recs, pattern, 20, min(8, n - len(cores))):
