@@ -1,0 +1,147 @@
+// Shared parts of bulge_kernel and bulge_index_kernel.
+#pragma once
+#include "kcommon.hpp"
+
+namespace cof {
+
+// ------------------------------------------------------- device bulge expansion
+//
+// bulge_kernel scores guides instead of compare keys: for each candidate it
+// takes the window at NS alignments (shift u: window position cb[u] at bit 0)
+// and, per guide, the mismatch masks M[u] of the guide at each alignment.  The
+// variants of the bulge family of shift u >= 1 (DNA or RNA bulge of one size,
+// every split j) count mm(j) = popc(left & bits[0,j)) + popc(right &
+// ~bits[0,j+gap)) with {left, right} = {M[u], M[0]}, so popc(M[u] & M[0]) - gap
+// is a lower bound of every mm(j) of the family: one AND and one popcount
+// filter the whole family.  Shift 0 alone is the no-bulge key (exact).  A
+// family that passes is bounded again on the two halves of its split range,
+// and only then are its splits counted one by one and emitted through the
+// family's variant entries (key index, threshold as bias; a key shared by
+// several variants is emitted by one of them).
+
+struct BulgeArgs {
+    DevGenome g;
+    PatternDesc pd;
+    int32_t cb[kMaxShifts];    // window position of mask bit 0 at each shift
+    uint32_t gap[kMaxShifts];  // RNA-bulge length of the family of each shift
+    uint32_t rev;              // reversed PAM: left part at shift 0, right part at shift u
+    uint32_t nunits;           // care classes
+    uint32_t tile_stride;
+    const uint32_t *table;
+    uint64_t limit;
+    uint64_t *hkey;
+    uint16_t *hmm;
+    unsigned long long *count;
+    unsigned long long *cand_count;
+    uint64_t cap;
+};
+
+// bits [0, x) of word w of an NW-word mask (x may be negative or past the end)
+__device__ __forceinline__ uint32_t low_bits(int32_t x, int w) {
+    const int32_t y = x - 32 * w;
+    return y <= 0 ? 0u : y >= 32 ? ~0u : (1u << y) - 1u;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t popc_below(const uint32_t (&m)[NW], int32_t x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) c += (uint32_t)__popc(m[w] & low_bits(x, w));
+    return c;
+}
+
+template <int NW>
+__device__ __forceinline__ uint32_t popc_from(const uint32_t (&m)[NW], int32_t x) {
+    uint32_t c = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++) c += (uint32_t)__popc(m[w] & ~low_bits(x, w));
+    return c;
+}
+
+// The variants of one family for lanes whose filter passed.  fb = the
+// family block [first split, split count, entry offset, segments, segment
+// records]; bias = the family filter bias -(max threshold + gap + 1).  Each
+// segment of splits [ja, jz] is bounded by popc(left below ja) +
+// popc(left & right in [ja, jz + gap)) + popc(right from jz + gap) - gap;
+// only the splits of a segment that passes are counted exactly.
+template <int NW>
+__device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32_t (&left)[NW],
+                                             const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
+                                             const TileSink &sink) {
+    constexpr int SW = bulge_seg_words(NW);
+    static_assert(kBulgeSegs == 4, "two halves of two segments");
+    const int32_t j0 = (int32_t)fb[0];
+    kptr_t ent = tab + fb[2];
+    // bound of the splits of segments [s0, s1]: left below the first split,
+    // left & right from it to the last split + gap (every position neither
+    // below nor above), right from there on.  An unused segment record is
+    // zero: the bound then counts fewer positions and stays a lower bound.
+    auto bound = [&](int s0, int s1) {
+        const kptr_t r0 = fb + 4 + s0 * SW, r1 = fb + 4 + s1 * SW;
+        uint32_t b = bias;
+#pragma unroll
+        for (int w = 0; w < NW; w++) {
+            const uint32_t below = r0[2 + w], above = r1[2 + 2 * NW + w];
+            b += (uint32_t)__popc(left[w] & below);
+            b += (uint32_t)__popc(__builtin_amdgcn_bitop3_b32(left[w], right[w], ~(below | above), kAnd3));
+            b += (uint32_t)__popc(right[w] & above);
+        }
+        return (int32_t)b;
+    };
+    // each half (two segments, an unused one has no splits) is bounded
+    // first, its segments only when the half passes
+#pragma unroll
+    for (int hf = 0; hf < 2; hf++) {
+        if (bound(2 * hf, 2 * hf + 1) >= 0) continue;
+#pragma unroll
+        for (int sg = 2 * hf; sg < 2 * hf + 2; sg++) {
+            const kptr_t sr = fb + 4 + sg * SW;
+            const uint32_t a = sr[0], cnt = sr[1];
+            if (cnt == 0 || bound(sg, sg) >= 0) continue;
+            for (uint32_t k = a; k < a + cnt; k++) {
+                const int32_t j = j0 + (int32_t)k;
+                const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
+                if ((int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+            }
+        }
+    }
+}
+
+// Window planes of candidate entry e (tile offset << 1 | strand) at the NS
+// alignments of the bulge table.
+template <int NW, int NS, typename Rd>
+__device__ __forceinline__ void bulge_windows(const BulgeArgs &a, Rd rd, uint32_t e, uint32_t (&l)[NS][NW],
+                                              uint32_t (&h)[NS][NW], uint32_t (&o)[NS][NW]) {
+    const uint32_t ls = e >> 1, strand = e & 1;
+#pragma unroll
+    for (int u = 0; u < NS; u++) {
+        const int64_t bit = strand == 0 ? (int64_t)ls + a.cb[u] + 32 * kHaloL
+                                        : (int64_t)ls + a.pd.P - 32 * NW - (int64_t)a.cb[u] + 32 * kHaloL;
+        window_planes<NW>(rd, bit, strand, l[u], h[u], o[u]);
+    }
+}
+
+template <int NW>
+__device__ __forceinline__ void class_planes(const uint32_t (&l)[NW], const uint32_t (&h)[NW], const uint32_t (&o)[NW],
+                                             uint32_t (&cls)[5][NW]) {
+#pragma unroll
+    for (int w = 0; w < NW; w++) {
+        cls[kClsT][w] = ~(l[w] | h[w] | o[w]);
+        cls[kClsC][w] = l[w] & ~h[w];
+        cls[kClsA][w] = h[w] & ~l[w];
+        cls[kClsG][w] = l[w] & h[w];
+        cls[kClsO][w] = o[w];
+    }
+}
+
+// Mismatch mask of a general (IUPAC) guide record against the class planes.
+template <int NW>
+__device__ __forceinline__ uint32_t general_mask(kptr_t kp, const uint32_t (&cls)[5][NW], int w) {
+    uint32_t x = cls[kClsO][w] & kp[kClsO * NW + w];
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsG][w], kp[kClsG * NW + w], x, kAndOr);
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsA][w], kp[kClsA * NW + w], x, kAndOr);
+    x = __builtin_amdgcn_bitop3_b32(cls[kClsC][w], kp[kClsC * NW + w], x, kAndOr);
+    return __builtin_amdgcn_bitop3_b32(cls[kClsT][w], kp[kClsT * NW + w], x, kAndOr);
+}
+
+}  // namespace cof
