@@ -225,7 +225,8 @@ class Context:
 
     def close(self):
         if self._p.value:
-            lib().cof_close(self._p)
+            if lib().cof_close(self._p):  # refused while a server is attached
+                raise CofError(self.error())
             self._p = C.c_void_p()
 
     def __enter__(self):
@@ -303,7 +304,10 @@ class Context:
 
     def __del__(self):
         if getattr(self, "_p", None) is not None and _lib is not None:
-            self.close()
+            try:
+                self.close()
+            except CofError:
+                pass
 
 
 class Server:

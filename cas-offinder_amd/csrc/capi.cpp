@@ -105,6 +105,7 @@ int cof_open(int device, cof_ctx **out) {
 
 int cof_close(cof_ctx *c) {
     if (!c) return COF_E_INVALID;
+    if (c->servers.load() > 0) return c->fail(COF_E_INVALID, "A server is attached to this context.");
     if (c->device < 0) {
         delete c;
         return COF_OK;
@@ -119,23 +120,29 @@ int cof_close(cof_ctx *c) {
 
 const char *cof_last_error(const cof_ctx *c) { return c ? c->err.c_str() : g_noctx_err.c_str(); }
 
+// A load that fails leaves the context's genome as it was; one that succeeds
+// replaces it (results searched before then no longer format).
 int cof_genome_load(cof_ctx *c, const char *path, int *nfiles) {
     if (!c || !path) return COF_E_INVALID;
+    if (c->servers.load() > 0) return c->fail(COF_E_INVALID, "A server is attached to this context.");
     if (c->device >= 0) (void)hipSetDevice(c->device);
     int nf = 0;
     std::string err;
     const char *v = getenv("COF_VERBOSE");
-    int rc = load_path(path, c->genome, nf, err, v && atoi(v) > 0);
-    if (rc) { c->genome_ready = false; return c->fail(rc, err); }
+    HostGenome g;
+    int rc = load_path(path, g, nf, err, v && atoi(v) > 0);
+    if (rc) return c->fail(rc, err);
     if (nfiles) *nfiles = nf;
-    return c->device < 0 ? COF_OK : c->upload_genome();
+    return c->install_genome(std::move(g));
 }
 
 int cof_genome_synth(cof_ctx *c, const cof_synth_spec *spec) {
     if (!c || !spec) return COF_E_INVALID;
+    if (c->servers.load() > 0) return c->fail(COF_E_INVALID, "A server is attached to this context.");
     if (c->device >= 0) (void)hipSetDevice(c->device);
-    synth_genome(*spec, c->genome);
-    return c->device < 0 ? COF_OK : c->upload_genome();
+    HostGenome g;
+    synth_genome(*spec, g);
+    return c->install_genome(std::move(g));
 }
 
 int cof_genome_info(const cof_ctx *c, uint64_t *total_len, uint64_t *n_files, uint64_t *n_contigs) {
@@ -271,6 +278,9 @@ int cof_result_records(const cof_result *r, cof_hit *out, size_t cap) {
 
 static int format_query(cof_ctx *c, const cof_result *r, int qi, std::string &text, SummaryTable *sum) {
     if (!c || !r || qi < 0 || (size_t)qi >= r->queries.size()) return COF_E_INVALID;
+    // hits index the genome they were searched on
+    if (r->genome_serial != c->genome_serial || c->genome.files.empty())
+        return c->fail(COF_E_INVALID, "The result was searched on another genome than the context's.");
     const size_t b = r->qbegin[qi], e = r->qbegin[qi + 1];
     static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
     format_hits(c->genome, *r->queries[qi], reinterpret_cast<const DevHit *>(r->hits.data()) + b, e - b, text, sum);

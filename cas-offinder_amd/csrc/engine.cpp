@@ -45,6 +45,10 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     std::vector<uint32_t> kw;
     std::vector<size_t> off;
     key_tables(qs, kw, off, true);
+    // the per-key tables overwrite d_keys: the lane's captured batched graph
+    // read the batched tables from it, so it must not be replayed without a
+    // new upload
+    L.graph_key.clear();
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
@@ -662,8 +666,17 @@ int cof_ctx::upload_genome() {
     return COF_OK;
 }
 
+int cof_ctx::install_genome(HostGenome &&g) {
+    static std::atomic<uint64_t> serials{0};
+    genome = std::move(g);
+    genome_serial = ++serials;
+    return device < 0 ? COF_OK : upload_genome();
+}
+
 int cof_ctx::search(const std::vector<std::shared_ptr<Query>> &qs, int mode, cof_result **out) {
     *out = nullptr;
     if (!genome_ready) return fail(COF_E_INVALID, "No genome loaded.");
-    return run_search(lane, dg, slab_positions, qs, mode, &tuner, out, err);
+    const int rc = run_search(lane, dg, slab_positions, qs, mode, &tuner, out, err);
+    if (rc == COF_OK) (*out)->genome_serial = genome_serial;
+    return rc;
 }

@@ -7,6 +7,7 @@
 #include <sys/mman.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstdlib>
 #include <map>
 #include <memory>
@@ -66,6 +67,7 @@ struct cof_result {
     std::vector<cof_hit, cof::UninitAlloc<cof_hit>> hits;  // sorted (query, key, pos, strand)
     std::vector<std::shared_ptr<cof::Query>> queries;
     std::vector<size_t> qbegin;                     // hits of query i: [qbegin[i], qbegin[i+1])
+    uint64_t genome_serial = 0;                     // the genome searched (cof_ctx::genome_serial)
 };
 
 namespace cof {
@@ -249,6 +251,11 @@ struct cof_ctx {
     cof::DevBuf<uint32_t> d_tile_pidx;
     cof::DevGenome dg{};
     bool genome_ready = false;
+    // Identifies the loaded genome: unique over all contexts of the process,
+    // renewed by every successful load or synth.  A result records the serial
+    // it was searched on and formats only against that genome.
+    uint64_t genome_serial = 0;
+    std::atomic<int> servers{0};  // attached servers: genome reloads and close are refused meanwhile
     cof::Lane lane;                        // the context's own lane
     cof::Tuner tuner;
     uint64_t slab_positions = 1ull << 31;  // plain path window starts per finder launch
@@ -258,5 +265,7 @@ struct cof_ctx {
         return code;
     }
     int upload_genome();
+    // Makes g the context's genome (host copy, device planes, new serial).
+    int install_genome(cof::HostGenome &&g);
     int search(const std::vector<std::shared_ptr<cof::Query>> &qs, int mode, cof_result **out);
 };

@@ -100,6 +100,7 @@ void cof_server::worker(Lane *L) {
             size_t q0 = 0;
             for (size_t i = 0; i < batch.size(); i++) {
                 auto *p = new cof_result();
+                p->genome_serial = ctx->genome_serial;  // reloads are refused while the server runs
                 const size_t nqi = batch[i]->qs.size();
                 p->queries = batch[i]->qs;
                 p->qbegin.assign(nqi + 1, 0);
@@ -166,6 +167,7 @@ int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out) 
         }
         s->lanes.push_back(std::move(L));
     }
+    ctx->servers++;
     for (auto &L : s->lanes) s->workers.emplace_back([s, p = L.get()] { s->worker(p); });
     *out = s;
     return COF_OK;
@@ -223,6 +225,7 @@ int cof_server_stop(cof_server *s) {
         delete kv.second->result;
         delete kv.second;
     }
+    s->ctx->servers--;
     delete s;
     return COF_OK;
 }

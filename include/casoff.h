@@ -119,7 +119,11 @@ int cof_device_count(int *n);
  *      non-2bit files skipped) or one file.  The whole genome is packed as
  *      2-bit bases + a non-ACGT bit plane and stays resident on the device
  *      until the next load or cof_close.  Returns the number of files loaded
- *      in *nfiles (may be NULL). */
+ *      in *nfiles (may be NULL).  A load that fails leaves the previous
+ *      genome in place; one that succeeds replaces it, and results searched
+ *      on the previous genome no longer format (COF_E_INVALID).  Loads,
+ *      cof_genome_synth and cof_close return COF_E_INVALID while a server
+ *      (cof_server_start) is attached to the context. */
 int cof_genome_load(cof_ctx *ctx, const char *path, int *nfiles);
 /* Generates the genome directly into the packed resident form. */
 int cof_genome_synth(cof_ctx *ctx, const cof_synth_spec *spec);
@@ -174,6 +178,8 @@ int cof_result_records(const cof_result *r, cof_hit *out, size_t cap);
  * --summary table (writeSummaryTable, :471-500) is written there. */
 int cof_result_format(cof_ctx *ctx, const cof_result *r, int qi, const char *path, int append,
                       const char *summary_path);
+/* Formatting reads the context's genome: a result formats only on the
+ * context and genome it was searched on (else COF_E_INVALID). */
 /* Same records into a malloc-ed buffer (caller frees with cof_free). */
 int cof_result_format_mem(cof_ctx *ctx, const cof_result *r, int qi, char **out, size_t *len);
 int cof_result_free(cof_result *r);

@@ -115,3 +115,68 @@ def oracle_records(tmp_path, input_text, name="in.txt", summary=False):
 
 def rng(seed):
     return random.Random(seed)
+
+
+# ---- genome regions: oracle runs on slices of a large resident genome
+
+class Piece:
+    """Positions [off, off + n) of contig `name` (index ci, global start
+    gpos, length clen) of a context's genome."""
+
+    def __init__(self, ctx, ci, off, n):
+        self.name, self.gpos, self.clen, _ = ctx.contigs()[ci]
+        self.ci, self.off = ci, off
+        self.n = min(n, self.clen - off)
+        assert self.n > 0 and off >= 0
+
+    def bounds(self, margin):
+        """Printed locations kept from this piece: records whose whole window
+        lies inside the piece on both sides.  A printed location is the
+        window start plus at most D + bulge size, so `margin` >= padded
+        pattern length + D + R keeps only windows that start and end inside;
+        a piece edge at a contig edge needs no margin (no window crosses it)."""
+        lo = self.off + (margin if self.off > 0 else 0)
+        hi = self.off + self.n - (margin if self.off + self.n < self.clen else 0)
+        return lo, hi
+
+
+def write_pieces(ctx, path, pieces, width=60):
+    """One FASTA record per piece, named as its contig."""
+    with open(path, "w") as f:
+        for p in pieces:
+            f.write(">" + p.name + "\n")
+            seq = ctx.text(p.gpos + p.off, p.n)
+            for i in range(0, len(seq), width):
+                f.write(seq[i:i + width] + "\n")
+
+
+def keep_in_pieces(lines, pieces, margin, rebase=False):
+    """Record lines whose printed location falls inside a piece's bounds; with
+    rebase the lines come from a run over write_pieces' FASTA, and their
+    locations are moved back to contig coordinates first."""
+    by_name = {p.name: p for p in pieces}
+    out = []
+    for l in lines:
+        f = l.split("\t")
+        p = by_name.get(f[4])
+        if p is None:
+            continue
+        loc = int(f[5]) + (p.off if rebase else 0)
+        lo, hi = p.bounds(margin)
+        if lo <= loc <= hi:
+            if rebase:
+                f[5] = str(loc)
+                l = "\t".join(f)
+            out.append(l)
+    return sorted(out)
+
+
+def hits_near_pieces(hits, pieces, margin):
+    """Hits whose window start lies within `margin` of a piece (global
+    coordinates): the only ones that can print a location inside it."""
+    import numpy as np
+    sel = np.zeros(len(hits), dtype=bool)
+    for p in pieces:
+        a = p.gpos + p.off
+        sel |= (hits["pos"] + margin >= a) & (hits["pos"] < a + p.n + margin)
+    return hits[sel]

@@ -106,3 +106,23 @@ def test_cli_arguments_without_search(tmp_path):
     assert p.returncode == 1 and b"No HIP devices found." in p.stderr
     p = run("in.txt", "G0:4:0", str(tmp_path / "o.txt"))
     assert p.returncode == 1 and b"device ID" in p.stderr
+
+
+def test_failed_load_leaves_the_genome(tmp_path):
+    """A load that fails (missing path) keeps the loaded genome; one that
+    succeeds replaces it."""
+    c = casoff.Context(-1)
+    ga, gb = str(tmp_path / "a.fa"), str(tmp_path / "b.fa")
+    with open(ga, "w") as f:
+        f.write(">a\nACGTACGAGG\n>b\nTTTT\n")
+    with open(gb, "w") as f:
+        f.write(">c\nGGGGGG\n")
+    c.load(ga)
+    info, contigs = c.info(), c.contigs()
+    with pytest.raises(casoff.CofError, match="opening directory"):
+        c.load(str(tmp_path / "missing"))
+    assert c.info() == info and c.contigs() == contigs
+    assert c.text(0, 15) == "ACGTACGAGG;TTTT"
+    c.load(gb)
+    assert [x[0] for x in c.contigs()] == ["c"] and c.text(0, 6) == "GGGGGG"
+    c.close()

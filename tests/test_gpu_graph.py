@@ -88,3 +88,33 @@ def test_plan_reused_across_searches_and_genomes(tmp_path):
                 assert len(got) == len(want) and (got == want).all(), (genome, sel)
     finally:
         ctx.close()
+
+
+def test_plain_search_between_batched_searches(tmp_path):
+    """A plain search writes its per-key tables into the lane's key buffer;
+    the batched search of the same query objects that follows must upload its
+    own tables again instead of replaying the captured graph over the plain
+    path's.  Bulge groups are in the set: their kernels take variant-block
+    offsets from the tables.  Every search equals the oracle."""
+    import helpers as H
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=23, n_contigs=4, total_len=1_200_000, n_frac=0.01), gp, 0)
+    texts = [t for _, t in workloads.group_inputs(workloads.mixed_guides(6, 40, 3, 6, 0.5, 1), gp)]
+    assert any(len(t.split("\n")[1].split()) == 3 for t in texts)  # some group has DNA/RNA bulges
+    want = [H.record_lines(H.oracle_records(tmp_path, t, name="in%d.txt" % i)) for i, t in enumerate(texts)]
+    ctx = _context(True)
+    try:
+        ctx.load(gp)
+        qs = [ctx.query(t) for t in texts]
+        modes = []
+        for mode in (casoff.MODE_BATCHED, casoff.MODE_PLAIN, casoff.MODE_BATCHED, casoff.MODE_PLAIN,
+                     casoff.MODE_BATCHED, casoff.MODE_BATCHED):
+            r = ctx.search(qs, mode)
+            modes.append((mode, ctx.stats()["graph"]))
+            for i in range(len(qs)):
+                assert H.record_lines(r.format(i)) == want[i], (mode, i)
+        # the batched search right after a plain one recaptured; the last one replayed
+        assert modes[2] == (casoff.MODE_BATCHED, 1) and modes[4] == (casoff.MODE_BATCHED, 1), modes
+        assert modes[5] == (casoff.MODE_BATCHED, 2), modes
+    finally:
+        ctx.close()
