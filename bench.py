@@ -2,13 +2,19 @@
 """Benchmark of the Cas-OFFinder search path on one MI355X.
 
 A step is one full search of the workload over the genome resident in HBM:
-key tables submitted -> fused finder+comparer kernels -> compact records in
-host memory (sorted).  Genome generation/upload and text formatting are not
-in the step and are reported separately.
+queries submitted -> fused finder+comparer kernels (one launch per pattern
+group) -> device sort -> compact records in host memory.  Genome
+generation/upload and text formatting are not in the step and are reported
+separately.
 
-Default workload (genome_3g_ngg): synthetic 3.1 Gbp genome (seed 1, 24
-contigs, GC 41%, ~5% N), pattern 20N+NGG, 100 random 20-nt guides, mm <= 4.
-Prints one JSON line.
+Default workload (mixed_pam_2000): the synthetic 3.1 Gbp genome (seed 1, 24
+contigs, GC 41%, ~5% N, 50% soft-masked), 2,000 guides of four Cas9 PAMs
+(seed 2) with thresholds 3-6 and 25% DNA/RNA bulges up to 1, one reference
+input file per (PAM, D, R) group, searched batched (the metric's value) and
+plain (one finder launch, then one comparer launch per compare key, the
+reference's launch model).  The 100-guide NGG and the 20-guide bulge
+workloads are reported beside it, then the continuous-batching server on a
+Poisson request stream and the CPU baseline.  Prints one JSON line.
 """
 import argparse
 import json
@@ -24,81 +30,61 @@ import casoff  # noqa: E402
 import workloads  # noqa: E402
 
 METRIC = "guide×Gbp/s bit-exact, batched vs plain on 1 GPU; serving p50/p99 latency"
-VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s: 256 CU x 64 lanes x 2.4 GHz
-HBM_PEAK_GBS = 8000.0
-# The batched kernels are bound by VALU issue (integer bit work; their HBM
-# reads are 0.375 B per genome position per launch).  roofline.achieved is
-# the dominant kernel's executed int32 lane-ops per launch (rocprofv3
-# SQ_INSTS_VALU x 64, from the committed summary of the same kernel and
-# workload) over its launch time measured here.  Beside it:
-#  - hbm: the algorithmic bytes (the three genome bit planes) per launch;
-#  - comparison_model (counted by the library as n_model_ops): fused_kernel
-#    (queries without bulges) 3 int32 ops per (candidate, compare key,
-#    32-position word) (2x v_bitop3 + v_bcnt; 6 for keys with IUPAC codes);
-#    bulge_kernel per (candidate, guide, word) 2 ops per window alignment (5
-#    with IUPAC codes), 1 for the no-bulge popcount and 2 (AND + popcount) per
-#    bulge family, plus one OR per family.  The seed-index kernels skip most
-#    of that work, so their frac in this unit exceeds 1;
-#  - onehot_cost_model: a 4-bit one-hot comparer (8 bases per 32-bit word,
-#    about 6 ops per word), i.e. 0.75 ops per base comparison of every
-#    expanded key with no early exit.
-ONEHOT_OPS_PER_BASE_CMP = 0.75
+HBM_PEAK_GBS = 8000.0      # MI355X HBM3E
+CU, CLOCK_HZ = 256, 2.4e9  # VALU issue peak: one wave64 instruction per CU-cycle
+# Algorithmic HBM bytes of one batched launch: every launch (one pattern
+# group) reads the genome's three bit planes once, 0.375 B per position
+# (2-bit base + 1-bit non-ACGT plane); candidates, windows and the key index
+# stay on chip (LDS / registers).  roofline.achieved = these bytes over the
+# dominant kernel's average launch time, measured here with HIP events around
+# each launch on the library's stream (cof_launch_info) over the timed steps.
+# The kernels are bound by VALU issue (integer bit work, no dense
+# contraction): roofline.valu gives the dominant kernel's executed VALU
+# instructions per launch (rocprofv3 SQ_INSTS_VALU, committed summary of the
+# same workload) over the issue peak of the same launch time.
+BYTES_PER_POSITION = 0.375
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_workload(name, ctx, n_guides=None):
+WORKLOADS = {
+    "genome_3g_ngg": "3.1 Gbp synthetic genome (seed 1, 24 contigs), pattern 20N+NGG, 100 guides, mm<=4",
+    "bulge_3g_nrg": "3.1 Gbp synthetic genome (seed 1), 20N+NRG, DNA bulge 2 / RNA bulge 1, 20 guides, mm<=4",
+    "mixed_pam_2000": "3.1 Gbp synthetic genome (seed 1), 2000 mixed-PAM guides (seed 2; SpCas9 NGG 40%, SaCas9 "
+                      "NNGRRT 30%, StCas9 NNAGAAW 20%, NmCas9 NNNNGMTT 10%; mm 3-6; 25% with DNA/RNA bulge <= 1), "
+                      "one reference input per (PAM, D, R) group: 16 groups",
+}
+
+
+def workload_texts(name):
     if name == "genome_3g_ngg":
-        texts = [workloads.genome_3g_ngg("synthetic", n_guides or 100)]
-        desc = "3.1 Gbp synthetic genome (seed 1, 24 contigs), pattern 20N+NGG, %d guides, mm<=4" % (n_guides or 100)
-    elif name == "bulge_3g_nrg":
-        texts = [workloads.bulge_3g_nrg("synthetic", n_guides or 20)]
-        desc = "3.1 Gbp synthetic genome (seed 1), 20N+NRG, DNA bulge 2 / RNA bulge 1, %d guides, mm<=4" % (n_guides or 20)
-    elif name == "mixed_pam_2000":
-        texts = [t for _, t in workloads.mixed_pam_2000("synthetic", n_guides or 2000)]
-        desc = "3.1 Gbp synthetic genome (seed 1), %d mixed-PAM guides (seed 2), one query per (PAM, D, R) group" % (n_guides or 2000)
-    else:
-        raise SystemExit("unknown workload " + name)
-    qs = [ctx.query(t) for t in texts]
-    guides = sum(len([l for l in t.split("\n")[2:] if l.strip()]) for t in texts)
-    return texts, qs, guides, desc
+        return [workloads.genome_3g_ngg("synthetic")]
+    if name == "bulge_3g_nrg":
+        return [workloads.bulge_3g_nrg("synthetic")]
+    if name == "mixed_pam_2000":
+        return [t for _, t in workloads.mixed_pam_2000("synthetic")]
+    raise SystemExit("unknown workload " + name)
 
 
-def kernels_of(variants):
-    """Kernels the batched path launched, from the autotuner's decisions
-    (signature -> variant): fused_kernel, fused_seed_kernel (variant 2) or
-    fused_mask_kernel (variants 3, 4) for key groups, bulge_kernel (variants
-    0, 1) or bulge_index_kernel (variant 2) for groups with DNA/RNA bulges."""
-    names = set()
-    for sig, v in variants.items():
-        if sig.startswith("bulge|"):
-            names.add("bulge_index_kernel" if v == 2 else "bulge_kernel")
-        else:
-            names.add({2: "fused_seed_kernel", 3: "fused_mask_kernel", 4: "fused_mask_kernel"}.get(v, "fused_kernel"))
-    return "+".join(sorted(names))
+def n_guides_of(texts):
+    return sum(len([l for l in t.split("\n")[2:] if l.strip()]) for t in texts)
 
 
-def profile_record(workload, kind):
-    """A figure per full-genome launch from a committed rocprofv3 summary of
-    this workload (profiles/*.md, an HTML comment `<!-- kind {json} -->`):
-    kind "traffic" (FETCH_SIZE / WRITE_SIZE bytes) or "valu" (SQ_INSTS_VALU
-    wave-instructions of the dominant kernel)."""
+def profile_kernel(workload, kernel):
+    """The committed rocprofv3 figures of one kernel instance on a workload
+    (profiles/*.md, HTML comments `<!-- kernel {json} -->`): average
+    duration, FETCH_SIZE / WRITE_SIZE bytes (scaled by the copy calibration)
+    and SQ_INSTS_VALU per launch."""
     import glob
     import re
     for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
-        m = re.search(r"<!-- %s (\{.*\}) -->" % kind, open(p).read())
-        if m:
+        for m in re.finditer(r"<!-- kernel (\{.*\}) -->", open(p).read()):
             d = json.loads(m.group(1))
-            if d.get("workload") == workload:
+            if d.get("workload") == workload and d.get("kernel") == kernel:
                 return d, os.path.relpath(p, ROOT)
     return None, None
-
-
-def measured_traffic(workload):
-    d, src = profile_record(workload, "traffic")
-    return (d["fetch_bytes"] + d["write_bytes"], src) if d else (None, None)
 
 
 def calibrate_copy(nbytes):
@@ -116,31 +102,127 @@ def calibrate_copy(nbytes):
 
 
 def time_steps(ctx, qs, mode, steps, warmup):
+    """Warmup searches (the first batched one runs the autotuner's trials and
+    grows the hit buffer), then `steps` timed searches.  Returns seconds per
+    step, kernel ms per step, launches per step, the last stats and result,
+    and per kernel instance the summed launch ms and launch count of the
+    timed steps."""
     tune_ms = 0.0
-    for _ in range(warmup):  # the first batched search runs the autotuner's trials
+    for _ in range(warmup):
         ctx.search(qs, mode)
         tune_ms += ctx.stats()["t_tune_ms"]
     time_steps.tune_ms = tune_ms
-    kernel_ms, launches = 0.0, 0
+    kernel_ms, launches, per_kernel, collect = 0.0, 0, {}, 0.0
     t0 = time.perf_counter()
     for _ in range(steps):
         r = ctx.search(qs, mode)
         st = ctx.stats()
         kernel_ms += st["t_fused_ms"] if mode == casoff.MODE_BATCHED else st["t_finder_ms"] + st["t_comparer_ms"]
         launches += st["n_launches"]
+        collect += st["t_collect_ms"]
+        if mode == casoff.MODE_BATCHED:
+            for rec in ctx.launch_info():
+                k = per_kernel.setdefault(rec["kernel"], {"ms": 0.0, "launches": 0})
+                k["ms"] += rec["ms"]
+                k["launches"] += 1
     t1 = time.perf_counter()
-    return (t1 - t0) / steps, kernel_ms / steps, launches / steps, st, r
+    time_steps.collect_ms = collect / steps
+    return (t1 - t0) / steps, kernel_ms / steps, launches / steps, st, r, per_kernel
 
 
-def cpu_baseline(ctx, texts, target_s=12.0):
-    """The oracle (CPU restatement, OpenMP) on a prefix of the first contig."""
+def roofline(workload, per_kernel, steps, genome_len):
+    """The dominant kernel instance (largest summed launch time over the timed
+    steps) against HBM, with its VALU issue rate beside it."""
+    kernel, rec = max(per_kernel.items(), key=lambda kv: kv[1]["ms"])
+    total_ms = sum(v["ms"] for v in per_kernel.values())
+    avg_ms = rec["ms"] / rec["launches"]
+    nbytes = int(BYTES_PER_POSITION * genome_len)
+    achieved = nbytes / (avg_ms * 1e-3) / 1e9
+    prof, src = profile_kernel(workload, kernel)
+    out = {"bound": "hbm", "kernel": kernel, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 4),
+           "traffic": int(prof["fetch_bytes"] + prof["write_bytes"]) if prof else None,
+           "algorithmic_bytes_per_launch": nbytes, "avg_launch_ms": round(avg_ms, 4),
+           "launches_per_step": rec["launches"] / steps,
+           "share_of_kernel_time": round(rec["ms"] / total_ms, 3), "profile": src,
+           "note": "algorithmic bytes = 0.375 B x genome positions (three bit planes read once per launch); "
+                   "avg_launch_ms from HIP events around each launch over the timed steps"}
+    if prof:
+        issue = prof["valu_insts"] / (CU * avg_ms * 1e-3 * CLOCK_HZ)
+        out["valu"] = {"insts_per_launch": prof["valu_insts"], "issue_per_cu_cycle": round(issue, 3),
+                       "peak_issue_per_cu_cycle": 1.0, "profile_avg_launch_ms": prof["avg_us"] / 1e3}
+    return out
+
+
+def run_workload(ctx, name, steps, warmup, plain_steps, with_plain=True):
+    texts = workload_texts(name)
+    qs = [ctx.query(t) for t in texts]
+    n_guides = n_guides_of(texts)
+    total = ctx.info()[0]
+    log("workload: %s (%d queries, %d guides)" % (name, len(qs), n_guides))
+    s_b, k_b, l_b, st_b, r_b, per_kernel = time_steps(ctx, qs, casoff.MODE_BATCHED, steps, warmup)
+    hits_b = r_b.records()
+    launches = [{k: rec[k] for k in ("kernel", "sig", "variant", "candidates")} | {"ms": round(rec["ms"], 3)}
+                for rec in ctx.launch_info()]
+    log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
+    res = {"value": round(n_guides * total / 1e9 / s_b, 2), "ms_per_step": round(s_b * 1e3, 3), "guides": n_guides,
+           "queries": len(qs), "kernel_ms_per_step": round(k_b, 3), "collect_ms_per_step": round(time_steps.collect_ms, 3),
+           "launches_per_step": l_b, "hits": len(hits_b),
+           "candidate_sites_per_s": st_b["n_pairs"] / s_b,
+           "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),
+                        "base_comparisons": st_b["n_base_cmp"]},
+           "autotune": {"trial_ms_in_warmup": round(time_steps.tune_ms, 2),
+                        "timed_steps_launch": {0: "eager launches", 1: "captured HIP graphs",
+                                               2: "replayed HIP graphs"}[int(st_b["graph"])]},
+           "launches": launches,
+           "roofline": roofline(name, per_kernel, steps, total)}
+    if with_plain:
+        s_p, k_p, l_p, st_p, r_p, _ = time_steps(ctx, qs, casoff.MODE_PLAIN, plain_steps, 0)
+        hits_p = r_p.records()
+        log("plain: %.2f ms/step (kernels %.2f ms), %d hits" % (s_p * 1e3, k_p, len(hits_p)))
+        same = bool(len(hits_b) == len(hits_p) and (hits_b == hits_p).all())
+        res["plain"] = {"value": round(n_guides * total / 1e9 / s_p, 2), "ms_per_step": round(s_p * 1e3, 3),
+                        "steps": plain_steps, "launches_per_step": l_p, "bit_exact_vs_batched": same}
+        res["batched_over_plain"] = round(s_p / s_b, 2)
+        if st_p["t_finder_ms"] > 0:
+            # plain path's finder (finder_count_kernel + tile-count scan +
+            # finder_write_kernel): 0.375 B read per scanned position + a
+            # (locus u32, flag u8) entry written per locus with a PAM
+            fbytes = 0.375 * st_p["n_finder_positions"] + 5 * st_p["n_loci"]
+            fach = fbytes / (st_p["t_finder_ms"] * 1e-3) / 1e9
+            res["plain"]["finder_roofline"] = {
+                "kernel": "finder_count_kernel + finder_write_kernel", "bound": "hbm",
+                "algorithmic_bytes_per_step": int(fbytes), "ms_per_step": round(st_p["t_finder_ms"], 4),
+                "achieved": round(fach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(fach / HBM_PEAK_GBS, 4)}
+    # cold step: fresh query objects (parse, tables and indexes built and
+    # uploaded, launches captured), tuner decisions of the warmup kept
+    t0 = time.perf_counter()
+    fresh = [ctx.query(t) for t in texts]
+    r = ctx.search(fresh, casoff.MODE_BATCHED)
+    res["cold_step_ms"] = round((time.perf_counter() - t0) * 1e3, 2)
+    assert r.count() == len(hits_b)
+    return texts, res
+
+
+def host_cores():
+    """Host cores this process may use: its CPU affinity, capped by
+    OMP_NUM_THREADS where the host sets it (the GPU host gives one GPU's
+    process a share of its cores)."""
+    n = len(os.sched_getaffinity(0))
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    return min(n, int(omp)) if omp.isdigit() and int(omp) > 0 else n
+
+
+def cpu_baseline(ctx, texts, target_s=15.0):
+    """The oracle (CPU restatement of the reference's finder/comparer, one
+    position / one candidate per iteration, OpenMP) over every query of the
+    workload on a prefix of the first contig, scaled to ~target_s of CPU work."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle
-    threads = min(os.cpu_count() or 1, 16)
+    threads = host_cores()
     name, pos, clen, _ = ctx.contigs()[0]
-    sample = min(clen, 4_000_000)
+    sample = min(clen, 1_000_000)
     with tempfile.TemporaryDirectory() as td:
-        res = None
         for _ in range(3):
             gp = os.path.join(td, "sample.fa")
             with open(gp, "w") as f:
@@ -148,19 +230,22 @@ def cpu_baseline(ctx, texts, target_s=12.0):
                 seq = ctx.text(pos, sample)
                 for i in range(0, len(seq), 60):
                     f.write(seq[i:i + 60] + "\n")
-            ip = os.path.join(td, "in.txt")
-            with open(ip, "w") as f:
-                f.write(gp + "\n" + texts[0].split("\n", 1)[1])
-            res = oracle.time_search(ip, threads)
-            if res["seconds"] >= target_s / 4 or sample >= clen:
+            secs, cands, glen = 0.0, 0, 0
+            for i, t in enumerate(texts):
+                ip = os.path.join(td, "in%d.txt" % i)
+                with open(ip, "w") as f:
+                    f.write(gp + "\n" + t.split("\n", 1)[1])
+                res = oracle.time_search(ip, threads)
+                secs += res["seconds"]
+                cands += res["candidates"]
+                glen = res["genome_len"]
+            if secs >= target_s / 3 or sample >= clen:
                 break
-            sample = min(clen, int(sample * max(2.0, target_s / max(res["seconds"], 1e-3))))
-    n_guides = len([l for l in texts[0].split("\n")[2:] if l.strip()])
-    value = n_guides * (res["genome_len"] / 1e9) / res["seconds"]
-    return {"value": value, "unit": "guide×Gbp/s", "cores": threads, "kind": "port",
-            "sample": "oracle (reference-algorithm CPU restatement, OpenMP) on the first %d bp of %s with the "
-                      "first query's %d guides: %.2f s, %d candidates" % (sample, name, n_guides, res["seconds"],
-                                                                          res["candidates"])}
+            sample = min(clen, int(sample * min(4.0, target_s / max(secs, 1e-3))))
+    n_guides = n_guides_of(texts)
+    return {"value": n_guides * (glen / 1e9) / secs, "unit": "guide×Gbp/s", "cores": threads, "kind": "port",
+            "sample": "oracle (reference-algorithm CPU restatement, OpenMP) over all %d queries (%d guides) on the "
+                      "first %d bp of %s: %.2f s, %d candidates" % (len(texts), n_guides, sample, name, secs, cands)}
 
 
 def serve_bench(ctx, n_requests, qps, seed=3, verify=True):
@@ -205,9 +290,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="genome_3g_ngg")
-    ap.add_argument("--guides", type=int, default=None)
-    ap.add_argument("--plain-steps", type=int, default=2)
+    ap.add_argument("--workload", default="mixed_pam_2000", choices=sorted(WORKLOADS))
+    ap.add_argument("--extra-workloads", default="genome_3g_ngg,bulge_3g_nrg",
+                    help="comma-separated workloads reported beside the main one ('' for none)")
+    ap.add_argument("--plain-steps", type=int, default=1)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--genome-len", type=int, default=3_100_000_000)
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
@@ -231,98 +317,40 @@ def main():
     t_load = time.perf_counter() - t0
     total, nfiles, ncontigs = ctx.info()
     log("genome: %d positions, %d contigs, generated+uploaded in %.1f s" % (total, ncontigs, t_load))
-    texts, qs, n_guides, desc = build_workload(args.workload, ctx, args.guides)
-
-    log("workload: %s (%d queries, %d guides)" % (args.workload, len(qs), n_guides))
-    s_b, k_b, l_b, st_b, r_b = time_steps(ctx, qs, casoff.MODE_BATCHED, args.steps, args.warmup)
-    hits_b = r_b.records()
-    tune = {"trial_ms_in_warmup": round(time_steps.tune_ms, 2), "variants": ctx.tuner_info(),
-            "timed_steps_launch": {0: "eager launches", 1: "captured HIP graph", 2: "replayed HIP graph"}[
-                int(st_b["graph"])]}
-    log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
-    if args.no_plain:
-        s_p, k_p, l_p, same = float("nan"), float("nan"), 0, None
-    else:
-        s_p, k_p, l_p, st_p, r_p = time_steps(ctx, qs, casoff.MODE_PLAIN, args.plain_steps, 0)
-        hits_p = r_p.records()
-        log("plain: %.2f ms/step (kernels %.2f ms), %d hits" % (s_p * 1e3, k_p, len(hits_p)))
-        same = bool(len(hits_b) == len(hits_p) and (hits_b == hits_p).all())
-
-    gbp = total / 1e9
-    value = n_guides * gbp / s_b
-    plain_value = n_guides * gbp / s_p if not args.no_plain else float("nan")
-    avg_launch_ms = k_b / max(l_b, 1)
-    launch_s = avg_launch_ms * 1e-3
-    model = st_b["n_model_ops"] / max(l_b, 1) / launch_s / 1e12
-    onehot = ONEHOT_OPS_PER_BASE_CMP * st_b["n_base_cmp"] / max(l_b, 1) / launch_s / 1e12
-    hbm_achieved = 0.375 * total / launch_s / 1e9
-    traffic, traffic_src = measured_traffic(args.workload)
-    kernel = kernels_of(tune["variants"])
-    # VALU: the lane-ops the dominant kernel executes per launch (rocprofv3
-    # SQ_INSTS_VALU x 64 lanes, committed summary of the same kernel and
-    # workload) over the launch time measured here; without such a summary,
-    # the comparison model's ops
-    vrec, vsrc = profile_record(args.workload, "valu")
-    if vrec and vrec.get("kernel") == kernel and l_b == 1:
-        valu_ops = vrec["valu_insts"] * 64
-        work = "%d int32 lane-ops executed per launch (rocprofv3 SQ_INSTS_VALU x 64, %s)" % (valu_ops, vsrc)
-    else:
-        valu_ops, vsrc = st_b["n_model_ops"] / max(l_b, 1), None
-        work = "%d int32 ops (%s)" % (valu_ops, {
-            "fused_kernel": "3 per (candidate, key, 32-position word)",
-            "bulge_kernel": "per (candidate, guide, word): 2 per alignment + 1 + 3 per bulge family",
-            "bulge_index_kernel": "bulge_kernel's family model, brute-force-equivalent",
-        }.get(kernel, "comparison-model units of the kernels summed"))
-    achieved = valu_ops / launch_s / 1e12
+    texts, main_res = run_workload(ctx, args.workload, args.steps, args.warmup, args.plain_steps, not args.no_plain)
+    main_res["autotune"]["variants"] = ctx.tuner_info()
     out = {
         "metric": METRIC,
-        "value": round(value, 2),
+        "value": main_res["value"],
         "unit": "guide×Gbp/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(s_b * 1e3, 3),
+        "ms_per_step": main_res["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
         "data": "synthetic",
-        "config": {"workload": args.workload + ": " + desc, "guides": n_guides, "genome_bp": total,
+        "config": {"workload": args.workload + ": " + WORKLOADS[args.workload], "guides": main_res["guides"],
+                   "queries": main_res["queries"], "genome_bp": total,
                    "mode": "batched (fused finder+comparer, one launch per pattern group)"},
-        "plain": None if args.no_plain else {
-            "value": round(plain_value, 2), "ms_per_step": round(s_p * 1e3, 3), "steps": args.plain_steps,
-            "launches_per_step": l_p, "bit_exact_vs_batched": same},
-        "batched_over_plain": None if args.no_plain else round(value / plain_value, 2),
-        "candidate_sites_per_s": st_b["n_pairs"] / s_b,
-        "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),
-                     "base_comparisons": st_b["n_base_cmp"]},
-        "genome_generate_upload_s": round(t_load, 2),
-        "autotune": tune,
-        "roofline": {"bound": "valu", "kernel": kernel, "achieved": round(achieved, 3),
-                     "peak": round(VALU_PEAK_TOPS, 2), "unit": "TOP/s", "frac": round(achieved / VALU_PEAK_TOPS, 4),
-                     "traffic": traffic, "traffic_source": traffic_src, "avg_launch_ms": round(avg_launch_ms, 4),
-                     "work_per_launch": work,
-                     "hbm": {"algorithmic_bytes_per_launch": int(0.375 * total), "achieved": round(hbm_achieved, 1),
-                             "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(hbm_achieved / HBM_PEAK_GBS, 4)},
-                     "comparison_model": {
-                         "ops_per_launch": int(st_b["n_model_ops"] / max(l_b, 1)), "achieved": round(model, 3),
-                         "frac": round(model / VALU_PEAK_TOPS, 4),
-                         "unit": "3 int32 ops per (candidate, key, 32-position word) for key tables, the bulge "
-                                 "kernel's family model for bulge tables: brute-force-equivalent work, which the "
-                                 "seed indexes skip (frac above 1)"},
-                     "onehot_cost_model": {"ops_per_base_cmp": ONEHOT_OPS_PER_BASE_CMP, "achieved": round(onehot, 2),
-                                           "frac": round(onehot / VALU_PEAK_TOPS, 3)}},
+        "roofline": main_res.pop("roofline"),
     }
-    if not args.no_plain and st_p["t_finder_ms"] > 0:
-        # plain path's finder (finder_count_kernel + tile-count scan +
-        # finder_write_kernel): algorithmic bytes = 0.375 B read per scanned
-        # position + a (locus u32, flag u8) entry written per locus with a PAM
-        fbytes = 0.375 * st_p["n_finder_positions"] + 5 * st_p["n_loci"]
-        fach = fbytes / (st_p["t_finder_ms"] * 1e-3) / 1e9
-        out["finder_roofline"] = {"kernel": "finder_count_kernel + finder_write_kernel (plain path)", "bound": "hbm",
-                                  "algorithmic_bytes_per_step": int(fbytes), "ms_per_step": round(st_p["t_finder_ms"], 4),
-                                  "achieved": round(fach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                                  "frac": round(fach / HBM_PEAK_GBS, 4)}
+    for k in ("plain", "batched_over_plain"):
+        if k in main_res:
+            out[k] = main_res.pop(k)
+    for k in ("value", "ms_per_step", "guides", "queries"):
+        main_res.pop(k)
+    out.update(main_res)
+    out["genome_generate_upload_s"] = round(t_load, 2)
+    extra = [w for w in args.extra_workloads.split(",") if w and w != args.workload]
+    if extra:
+        out["workloads"] = {}
+        for w in extra:
+            _, res = run_workload(ctx, w, args.steps, args.warmup, max(args.plain_steps, 1), not args.no_plain)
+            res.pop("launches")
+            out["workloads"][w] = res
     if args.serve_requests > 0:
         log("serving: %d requests at %.1f QPS" % (args.serve_requests, args.serve_qps))
         out["serving"] = serve_bench(ctx, args.serve_requests, args.serve_qps)

@@ -98,6 +98,7 @@ def lib():
             "cof_search": (i32, [vp, C.POINTER(vp), i32, i32, C.POINTER(vp)]),
             "cof_stats_get": (i32, [vp, C.POINTER(Stats)]),
             "cof_tuner_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
+            "cof_launch_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_result_count": (C.c_int64, [vp]),
             "cof_result_records": (i32, [vp, vp, sz]),
             "cof_result_format": (i32, [vp, vp, i32, C.c_char_p, i32, C.c_char_p]),
@@ -292,6 +293,17 @@ class Context:
             raise CofError(self.error())
         text = _take_buffer(out, ln.value)
         return dict((l.rsplit("\t", 1)[0], int(l.rsplit("\t", 1)[1])) for l in text.splitlines() if l)
+
+    def launch_info(self):
+        """Batched launches of the last search: [{kernel, sig, variant, ms, candidates}]."""
+        out, ln = C.c_void_p(), C.c_size_t()
+        if lib().cof_launch_info(self._p, C.byref(out), C.byref(ln)):
+            raise CofError(self.error())
+        recs = []
+        for l in _take_buffer(out, ln.value).splitlines():
+            k, sig, v, ms, cand = l.split("\t")
+            recs.append({"kernel": k, "sig": sig, "variant": int(v), "ms": float(ms), "candidates": int(cand)})
+        return recs
 
     def format_hits(self, query, hits):
         hits = np.ascontiguousarray(hits, dtype=HIT_DTYPE)

@@ -267,6 +267,22 @@ int cof_tuner_info(cof_ctx *c, char **out, size_t *len) {
     return COF_OK;
 }
 
+int cof_launch_info(const cof_ctx *c, char **out, size_t *len) {
+    if (!c || !out || !len) return COF_E_INVALID;
+    std::string text;
+    char buf[64];
+    for (const LaunchRec &lr : c->lane.launches) {
+        snprintf(buf, sizeof buf, "\t%d\t%.6f\t%llu\n", lr.variant, lr.ms, (unsigned long long)lr.candidates);
+        text += lr.kernel + "\t" + lr.sig + buf;
+    }
+    *out = static_cast<char *>(malloc(text.size() + 1));
+    if (!*out) return COF_E_NOMEM;
+    memcpy(*out, text.data(), text.size());
+    (*out)[text.size()] = 0;
+    *len = text.size();
+    return COF_OK;
+}
+
 int64_t cof_result_count(const cof_result *r) { return r ? (int64_t)r->hits.size() : COF_E_INVALID; }
 
 int cof_result_records(const cof_result *r, cof_hit *out, size_t cap) {

@@ -49,6 +49,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     // read the batched tables from it, so it must not be replayed without a
     // new upload
     L.graph_key.clear();
+    L.launches.clear();
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
@@ -329,7 +330,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         }
         put(kw.data(), kw.size() * 4);
     }
-    const bool replay = L.graph && L.graphs && gkey == L.graph_key;
+    const bool replay = !L.graph.empty() && L.graphs && gkey == L.graph_key;
     if (!replay && !kw.empty())
         HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
@@ -415,40 +416,58 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                     "hipMemsetAsync");
         t_tune = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
     }
-    HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
+    while (L.lev.size() < nlaunch + 1) {
+        hipEvent_t e = nullptr;
+        HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+        L.lev.push_back(e);
+    }
+    std::vector<std::string> kernels(nlaunch);
     if (!replay && L.graphs && nlaunch) {
-        // capture the launch sequence (kernels only: counters are cleared
-        // before, outside the graph); a failed capture falls back to eager
-        // launches for the rest of the lane's life
-        if (L.graph) (void)hipGraphExecDestroy(L.graph);
-        L.graph = nullptr;
-        L.graph_key.clear();
-        hipGraph_t g = nullptr;
-        hipError_t e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal);
-        for (size_t i = 0; e == hipSuccess && i < nlaunch; i++) e = launch(i, variant[i], 1, 0);
-        hipError_t e2 = hipStreamEndCapture(L.stream, &g);
-        if (e == hipSuccess && e2 == hipSuccess && g)
-            e = hipGraphInstantiate(&L.graph, g, nullptr, nullptr, 0);
-        else if (e == hipSuccess)
-            e = e2 != hipSuccess ? e2 : hipErrorUnknown;
-        if (g) (void)hipGraphDestroy(g);
+        // capture each launch as a graph of its own (kernels only: counters
+        // are cleared before, outside the graphs; the events between them
+        // time each launch); a failed capture falls back to eager launches
+        // for the rest of the lane's life
+        L.drop_graphs();
+        hipError_t e = hipSuccess;
+        for (size_t i = 0; e == hipSuccess && i < nlaunch; i++) {
+            hipGraph_t g = nullptr;
+            hipGraphExec_t x = nullptr;
+            reset_launched_kernel();
+            e = hipStreamBeginCapture(L.stream, hipStreamCaptureModeThreadLocal);
+            if (e != hipSuccess) break;
+            e = launch(i, variant[i], 1, 0);
+            kernels[i] = last_launched_kernel();
+            const hipError_t e2 = hipStreamEndCapture(L.stream, &g);
+            if (e == hipSuccess && e2 == hipSuccess && g) e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+            else if (e == hipSuccess) e = e2 != hipSuccess ? e2 : hipErrorUnknown;
+            if (g) (void)hipGraphDestroy(g);
+            if (e == hipSuccess) L.graph.push_back(x);
+        }
         if (e == hipSuccess) {
             L.graph_key = gkey;
             L.graph_variants = variant;
+            L.graph_kernels = kernels;
         } else {
             (void)hipGetLastError();
-            L.graph = nullptr;
+            L.drop_graphs();
             L.graphs = false;
         }
     }
-    L.stats.graph = 0;
-    if (L.graph && L.graphs && L.graph_key == gkey) {
-        HIP_TRY(hipGraphLaunch(L.graph, L.stream), "hipGraphLaunch");
-        L.stats.graph = replay ? 2 : 1;
-    } else {
-        for (size_t i = 0; i < nlaunch; i++)
+    const bool use_graph = !L.graph.empty() && L.graphs && L.graph_key == gkey;
+    L.stats.graph = use_graph ? (replay ? 2 : 1) : 0;
+    HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
+    for (size_t i = 0; i < nlaunch; i++) {
+        HIP_TRY(hipEventRecord(L.lev[i], L.stream), "hipEventRecord");
+        if (use_graph) {
+            HIP_TRY(hipGraphLaunch(L.graph[i], L.stream), "hipGraphLaunch");
+            kernels[i] = L.graph_kernels[i];
+        } else {
+            reset_launched_kernel();
             HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+            kernels[i] = last_launched_kernel();
+        }
     }
+    HIP_TRY(hipEventRecord(L.lev[nlaunch], L.stream), "hipEventRecord");
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
     HIP_TRY(hipMemcpyAsync(raw.data(), L.d_count.p, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -462,6 +481,15 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_fused_ms = ms_between(L.ev[0], L.ev[1]);
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
+    L.launches.assign(nlaunch, LaunchRec());
+    for (size_t i = 0; i < nlaunch; i++) {
+        LaunchRec &lr = L.launches[i];
+        lr.kernel = kernels[i];
+        lr.sig = sig[i];
+        lr.variant = variant[i];
+        lr.ms = ms_between(L.lev[i], L.lev[i + 1]);
+        lr.candidates = cnt[2 + i];
+    }
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
     for (size_t g = 0; g < ktab.size(); g++) {
         const uint64_t c = cnt[2 + g];
@@ -504,11 +532,18 @@ hipError_t Lane::init() {
     return hipSuccess;
 }
 
+void Lane::drop_graphs() {
+    for (hipGraphExec_t x : graph) (void)hipGraphExecDestroy(x);
+    graph.clear();
+    graph_key.clear();
+    graph_kernels.clear();
+}
+
 void Lane::release() {
     if (stream) (void)hipStreamSynchronize(stream);
-    if (graph) (void)hipGraphExecDestroy(graph);
-    graph = nullptr;
-    graph_key.clear();
+    drop_graphs();
+    for (hipEvent_t x : lev) (void)hipEventDestroy(x);
+    lev.clear();
     plan.reset();
     d_hkey.release();
     d_hkey2.release();

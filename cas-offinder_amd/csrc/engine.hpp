@@ -116,6 +116,21 @@ hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns,
                               uint32_t max_tiles);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
                      uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
+// Kernel instance of the calling thread's last batched launch, as profilers
+// name it ("cof::fused_mask_kernel<20, 3, 8, 2>"); empty after a reset when
+// the launcher had nothing to launch.
+const char *last_launched_kernel();
+void reset_launched_kernel();
+
+// One batched launch of a search: kernel instance, launch signature, variant,
+// duration (HIP events around the launch on the lane's stream) and the
+// (locus, strand) candidates it scanned.
+struct LaunchRec {
+    std::string kernel, sig;
+    int variant = 0;
+    double ms = 0;
+    uint64_t candidates = 0;
+};
 
 // Pinned host buffer (hipHostMalloc), grow-only.
 template <typename T>
@@ -193,12 +208,17 @@ struct Lane {
     DevBuf<uint8_t> d_scan_tmp;
     DevBuf<uint32_t> d_keys;
     hipEvent_t ev[6] = {};
+    std::vector<hipEvent_t> lev;       // around each batched launch
     cof_stats stats{};
-    // batched launch sequence of the last search as a HIP graph, replayed when
-    // a search has the same tables, kernel arguments and buffers
-    hipGraphExec_t graph = nullptr;
+    std::vector<LaunchRec> launches;   // batched launches of the last search
+    // batched launch sequence of the last search, one HIP graph per launch
+    // (events between them time each launch), replayed when a search has the
+    // same tables, kernel arguments and buffers
+    std::vector<hipGraphExec_t> graph;
     std::string graph_key;
     std::vector<int> graph_variants;
+    std::vector<std::string> graph_kernels;
+    void drop_graphs();
     bool graphs = true;
     bool trials = true;  // may run autotuner trials (off where latency matters: serving)
     std::shared_ptr<BatchPlan> plan;  // of the last batched search

@@ -172,6 +172,7 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
 
 template <int NW, bool GEN, int NS>
 static void bulge_variant(int variant, dim3 grid, const BulgeArgs &a, hipStream_t s) {
+    note_kernel("bulge_kernel", NW, GEN, NS, variant != 1);
     if (variant == 1) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, false>), grid, dim3(kThreads), 0, s, a);
     else hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, true>), grid, dim3(kThreads), 0, s, a);
 }

@@ -115,6 +115,7 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
 
 template <int NK, int NS>
 static hipError_t bulge_index_parts(uint32_t parts, dim3 grid, const BulgeIndexArgs &a, uint32_t lds, hipStream_t s) {
+    note_kernel("bulge_index_kernel", NK, NS, parts <= 4 ? 4 : parts <= 6 ? 6 : 8);
     if (parts <= 4) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 4>), grid, dim3(kThreads), lds, s, a);
     else if (parts <= 6) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 6>), grid, dim3(kThreads), lds, s, a);
     else hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 8>), grid, dim3(kThreads), lds, s, a);

@@ -130,6 +130,7 @@ __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
 
 template <int NW, bool GEN>
 static void fused_variant(int variant, dim3 grid, const FusedArgs &a, hipStream_t s) {
+    note_kernel("fused_kernel", NW, GEN, variant == 1 ? 8 : 32);
     if (variant == 1) hipLaunchKernelGGL((fused_kernel<NW, GEN, 8>), grid, dim3(kThreads), 0, s, a);
     else hipLaunchKernelGGL((fused_kernel<NW, GEN, 32>), grid, dim3(kThreads), 0, s, a);
 }
@@ -183,6 +184,7 @@ hipError_t launch_seed(const DevGenome &g, const PatternDesc &pd, const SeedDesc
     a.cand_count = cand_count;
     a.cap = cap;
     const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    note_kernel("fused_seed_kernel");
     hipLaunchKernelGGL(fused_seed_kernel, grid, dim3(kThreads), (lds_bytes + 15) / 16 * 16, s, a);
     return hipGetLastError();
 }

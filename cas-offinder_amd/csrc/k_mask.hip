@@ -160,6 +160,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
 template <int NK, int MINEX, int NP, int G>
 static hipError_t mask_launch(const MaskArgs &a, uint32_t lds, hipStream_t s) {
     const dim3 grid((a.ntile + G - 1) / G);
+    note_kernel("fused_mask_kernel", NK, MINEX, NP, G);
     hipLaunchKernelGGL((fused_mask_kernel<NK, MINEX, NP, G>), grid, dim3(kThreads * G), lds, s, a);
     return hipGetLastError();
 }

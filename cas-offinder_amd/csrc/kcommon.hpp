@@ -31,10 +31,29 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <string>
 
 #include "devtypes.hpp"
 
 namespace cof {
+
+// The kernel instance that the calling thread launched last, named as
+// profilers print it ("cof::bulge_kernel<1, false, 2, true>"); the engine
+// reads it after each batched launch (last_launched_kernel).
+inline thread_local std::string g_kernel_name;
+inline std::string kname_arg(int v) { return std::to_string(v); }
+inline std::string kname_arg(bool v) { return v ? "true" : "false"; }
+template <typename... A>
+inline void note_kernel(const char *name, A... a) {
+    std::string s = std::string("cof::") + name;
+    if constexpr (sizeof...(A) > 0) {
+        const std::string args[] = {kname_arg(a)...};
+        s += "<";
+        for (size_t i = 0; i < sizeof...(A); i++) s += (i ? ", " : "") + args[i];
+        s += ">";
+    }
+    g_kernel_name = std::move(s);
+}
 
 namespace {
 

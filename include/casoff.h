@@ -168,6 +168,15 @@ int cof_stats_get(const cof_ctx *ctx, cof_stats *out);
  * Caller frees *out with cof_free. */
 int cof_tuner_info(cof_ctx *ctx, char **out, size_t *len);
 
+/* Batched launches of the last cof_search on the context, one line each:
+ * "kernel<TAB>signature<TAB>variant<TAB>ms<TAB>candidates": the kernel
+ * instance as profilers name it, the autotuner's launch signature and
+ * variant, the launch's duration (HIP events around it on the context's
+ * stream) and the (locus, strand) candidates it scanned.  Empty after a
+ * plain search.  No reference counterpart (its kernels are launched without
+ * profiling, cas-offinder.cpp:123).  Caller frees *out with cof_free. */
+int cof_launch_info(const cof_ctx *ctx, char **out, size_t *len);
+
 /* ---- results */
 int64_t cof_result_count(const cof_result *r);
 int cof_result_records(const cof_result *r, cof_hit *out, size_t cap);

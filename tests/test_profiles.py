@@ -10,7 +10,8 @@ import re
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 WORKLOADS = {"genome_3g_ngg", "bulge_3g_nrg", "mixed_pam_2000"}
-KERNELS = {"fused_kernel", "fused_seed_kernel", "fused_mask_kernel", "bulge_kernel", "bulge_index_kernel"}
+KERNELS = {"fused_kernel", "fused_seed_kernel", "fused_mask_kernel", "bulge_kernel", "bulge_index_kernel",
+           "bulge_slice_kernel"}
 
 
 def _bench():
@@ -20,33 +21,30 @@ def _bench():
     return mod
 
 
-def _records(kind):
+def _records():
     out = []
     for p in sorted(glob.glob(os.path.join(ROOT, "profiles", "*.md"))):
-        for m in re.finditer(r"<!-- %s (\{.*\}) -->" % kind, open(p).read()):
+        for m in re.finditer(r"<!-- kernel (\{.*\}) -->", open(p).read()):
             out.append((p, json.loads(m.group(1))))
     return out
 
 
-def test_traffic_records():
-    recs = _records("traffic")
-    assert {r["workload"] for _, r in recs} == WORKLOADS
-    for p, r in recs:
-        assert set(r["kernel"].split("+")) <= KERNELS, p
-        # a full-genome launch reads at least the 1.1625 GB of bit planes once
-        assert r["fetch_bytes"] >= 1.1625e9 and r["write_bytes"] > 0, p
-
-
-def test_valu_records_and_bench_lookup():
+def test_kernel_records_and_bench_lookup():
+    """Every per-kernel record names a workload bench.py runs and a kernel
+    instance of the library, carries positive per-launch figures (a
+    full-genome launch reads at least the 1.1625 GB of bit planes once), and
+    is what bench.py finds for that (workload, kernel)."""
     bench = _bench()
-    recs = _records("valu")
-    assert "genome_3g_ngg" in {r["workload"] for _, r in recs}
+    recs = _records()
+    assert "mixed_pam_2000" in {r["workload"] for _, r in recs}
     for p, r in recs:
-        assert r["kernel"] in KERNELS and r["kernel_instance"].startswith("cof::" + r["kernel"]), p
-        assert r["valu_insts"] > 0, p
-        got, src = bench.profile_record(r["workload"], "valu")
+        assert r["workload"] in WORKLOADS, p
+        assert r["kernel"].startswith("cof::") and r["kernel"].split("<")[0][5:] in KERNELS, p
+        assert r["avg_us"] > 0 and r["valu_insts"] > 0, p
+        assert r["fetch_bytes"] >= 1.1625e9 and r["write_bytes"] > 0, p
+        got, src = bench.profile_kernel(r["workload"], r["kernel"])
         assert got == r and src == os.path.relpath(p, ROOT)
-    assert bench.profile_record("no_such_workload", "valu") == (None, None)
+    assert bench.profile_kernel("no_such_workload", "cof::x") == (None, None)
 
 
 def test_summaries_quote_their_commands():
