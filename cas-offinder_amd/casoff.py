@@ -99,6 +99,8 @@ def lib():
             "cof_stats_get": (i32, [vp, C.POINTER(Stats)]),
             "cof_tuner_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_launch_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
+            "cof_tuner_load": (i32, [vp, C.c_char_p]),
+            "cof_tuner_save": (i32, [vp, C.c_char_p]),
             "cof_result_count": (C.c_int64, [vp]),
             "cof_result_records": (i32, [vp, vp, sz]),
             "cof_result_format": (i32, [vp, vp, i32, C.c_char_p, i32, C.c_char_p]),
@@ -293,6 +295,14 @@ class Context:
             raise CofError(self.error())
         text = _take_buffer(out, ln.value)
         return dict((l.rsplit("\t", 1)[0], int(l.rsplit("\t", 1)[1])) for l in text.splitlines() if l)
+
+    def tuner_load(self, path):
+        if lib().cof_tuner_load(self._p, path.encode()):
+            raise CofError(self.error())
+
+    def tuner_save(self, path):
+        if lib().cof_tuner_save(self._p, path.encode()):
+            raise CofError(self.error())
 
     def launch_info(self):
         """Batched launches of the last search: [{kernel, sig, variant, ms, candidates}]."""

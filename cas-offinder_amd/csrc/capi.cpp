@@ -1,4 +1,6 @@
 // extern "C" entry points of libcasoff.so (declared in include/casoff.h).
+#include <dlfcn.h>
+
 #include <algorithm>
 #include <cstdio>
 #include <cstdlib>
@@ -12,6 +14,21 @@
 using namespace cof;
 
 static thread_local std::string g_noctx_err;
+
+// The tuner file a context starts from: $COF_TUNER_FILE ("" or "none": no
+// file), else tuned_gfx950.json beside the library when it exists.
+static std::string default_tuner_file() {
+    if (const char *e = getenv("COF_TUNER_FILE")) return strcmp(e, "none") == 0 ? std::string() : std::string(e);
+    Dl_info info{};
+    if (!dladdr((void *)&default_tuner_file, &info) || !info.dli_fname) return std::string();
+    std::string p = info.dli_fname;
+    const size_t sl = p.rfind('/');
+    p = (sl == std::string::npos ? std::string(".") : p.substr(0, sl)) + "/tuned_gfx950.json";
+    FILE *f = fopen(p.c_str(), "rb");
+    if (!f) return std::string();
+    fclose(f);
+    return p;
+}
 
 extern "C" {
 
@@ -72,9 +89,14 @@ int cof_device_ids(const char *devarg, int n_devices, uint8_t *selected, int *n_
 int cof_open(int device, cof_ctx **out) {
     if (!out) return COF_E_INVALID;
     *out = nullptr;
+    const std::string tf = default_tuner_file();
     if (device == -1) {  // host-only context
         auto *c = new cof_ctx();
         c->device = -1;
+        if (!tf.empty() && c->tuner.load(tf, g_noctx_err)) {
+            delete c;
+            return COF_E_INVALID;
+        }
         *out = c;
         return COF_OK;
     }
@@ -93,6 +115,10 @@ int cof_open(int device, cof_ctx **out) {
     if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
     if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = std::min(std::max(atoi(fv), 0), cof::kFusedVariants - 1);
     if (const char *gr = getenv("COF_GRAPH")) c->lane.graphs = strcmp(gr, "0") != 0;
+    if (!tf.empty() && c->tuner.load(tf, g_noctx_err)) {
+        delete c;
+        return COF_E_INVALID;
+    }
     if (c->lane.init() != hipSuccess) {
         c->lane.release();
         delete c;
@@ -265,6 +291,20 @@ int cof_tuner_info(cof_ctx *c, char **out, size_t *len) {
     (*out)[text.size()] = 0;
     *len = text.size();
     return COF_OK;
+}
+
+int cof_tuner_load(cof_ctx *c, const char *path) {
+    if (!c || !path) return COF_E_INVALID;
+    std::string err;
+    const int rc = c->tuner.load(path, err);
+    return rc ? c->fail(rc, err) : COF_OK;
+}
+
+int cof_tuner_save(cof_ctx *c, const char *path) {
+    if (!c || !path) return COF_E_INVALID;
+    std::string err;
+    const int rc = c->tuner.save(path, err);
+    return rc ? c->fail(rc, err) : COF_OK;
 }
 
 int cof_launch_info(const cof_ctx *c, char **out, size_t *len) {

@@ -66,6 +66,43 @@ constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that neve
 constexpr int kBulgeSlotWords = 3 + kMaxShifts;
 constexpr uint32_t kBulgeIndexMaxBytes = 32 * 1024;  // static LDS + index stays below 64 KiB
 
+// Bit-sliced bulge filter (bulge_slice_kernel): the guides of a single-class
+// A/C/G/T bulge table with a contiguous care run of kn <= 32 positions, as
+// 32-guide words.  Per word a block of wstride dwords: GL[KN], GH[KN] (bit g:
+// the lo / hi code bit of slot 32 * word + g at the k-th care position), then
+// per filter family f (dn window shifts -1..-dn, then dp shifts +1..+dp) six
+// threshold planes T[0..5] and an active word: family f's count
+// popc(M0 & Mf) over the care run must be at most T(g) for slot g to reach
+// the exact counts; then a flags dword (bit 0: the word's bounds are large
+// enough that each family is bounded on two halves of its split range
+// instead of the whole range).  Slots are ordered by decreasing bound, so
+// that a word holds guides of similar thresholds.  After the blocks, per slot
+// kBulgeSlotWords words [glo, ghi, variant block offset, family filter bias
+// per alignment].
+constexpr int kSliceTPlanes = 6;
+constexpr int kSliceFineT = 6;  // smallest bound of a word bounded on halves
+constexpr uint32_t kSliceMaxGuides = 256;
+struct SliceDesc {
+    uint32_t nslots, nwords;  // guides (slots) and 32-slot words
+    uint32_t kn, KN;          // care positions, the kernel's position count (kn in (KN - 4, KN])
+    uint32_t dn, dp;          // negative / positive window shifts of the filter families
+    int32_t fbase;            // table-frame window position of frame bit 0
+    uint32_t wstride;         // dwords per word block (a multiple of 4)
+    uint32_t slot_off;        // first dword of the slot records
+    uint32_t care;            // care mask of the class (table frame), for the exact counts
+};
+// What bulge_slice_kernel can run (the builder and the launcher both check
+// it): at most two filter families, one per alignment of a table of ns, a
+// care run of kn positions in (KN - 4, KN] for a kernel width KN of 16-32,
+// word blocks holding the planes and bounds, slots after the blocks.
+inline bool slice_desc_valid(const SliceDesc &sd, int ns) {
+    const bool width = sd.KN == 16 || sd.KN == 20 || sd.KN == 24 || sd.KN == 28 || sd.KN == 32;
+    return ns >= 2 && ns <= 3 && sd.dn <= 2 && sd.dp <= 2 && (int)(sd.dn + sd.dp) == ns - 1 && width &&
+           sd.kn <= sd.KN && sd.kn + 4 > sd.KN && sd.nslots >= 1 && sd.nslots <= kSliceMaxGuides &&
+           sd.nwords == (sd.nslots + 31) / 32 && sd.wstride % 4 == 0 &&
+           sd.wstride >= 2 * sd.KN + (kSliceTPlanes + 1) * (sd.dn + sd.dp) + 1 && sd.slot_off == sd.nwords * sd.wstride;
+}
+
 // Statistics counters updated once per workgroup are spread over kCountSlots
 // slots kSlotStride words apart (one 128-byte line each): a single hot address
 // serializes its atomics in one L2 channel.  The host sums the slots.

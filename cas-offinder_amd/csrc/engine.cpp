@@ -227,6 +227,9 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
     bidx.assign(btab.size(), MaskIndex());
     has_bidx.assign(btab.size(), 0);
     bioff.assign(btab.size(), 0);
+    P.bslice.assign(btab.size(), SliceIndex());
+    P.has_bslice.assign(btab.size(), 0);
+    P.bsoff.assign(btab.size(), 0);
     for (size_t g = 0; g < btab.size(); g++) {
         boff[g] = kw.size();
         kw.insert(kw.end(), btab[g].words.begin(), btab[g].words.end());
@@ -236,6 +239,11 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
             bioff[g] = kw.size();
             kw.insert(kw.end(), bidx[g].words.begin(), bidx[g].words.end());
             kw.resize((kw.size() + 3) & ~(size_t)3, 0);
+        }
+        P.has_bslice[g] = build_bulge_slices(btab[g], btab[g].pat, P.bslice[g]);
+        if (P.has_bslice[g]) {
+            P.bsoff[g] = kw.size();
+            kw.insert(kw.end(), P.bslice[g].words.begin(), P.bslice[g].words.end());
         }
     }
     const size_t nlaunch = ktab.size() + btab.size();
@@ -299,6 +307,9 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     const auto &has_bidx = P.has_bidx;
     const auto &bioff = P.bioff;
     const auto &sig = P.sig;
+    const auto &bslice = P.bslice;
+    const auto &has_bslice = P.has_bslice;
+    const auto &bsoff = P.bsoff;
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
     // identity of the launch sequence: every kernel argument and the tables
@@ -327,6 +338,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(&t.pat, sizeof(PatternDesc));
             put(&boff[g], sizeof boff[g]);
             put(&bioff[g], sizeof bioff[g]);
+            put(&bsoff[g], sizeof bsoff[g]);
         }
         put(kw.data(), kw.size() * 4);
     }
@@ -352,6 +364,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                                 cand_slots(i), L.d_hkey.n, L.stream, v, stride, max_tiles);
         const size_t bg = i - ktab.size();
         const BulgeTable &t = btab[bg];
+        if (v == 3 && has_bslice[bg])
+            return launch_bulge_slice(dg, t.pat, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg], bslice[bg].desc,
+                                      L.d_keys.p + bsoff[bg], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
+                                      L.d_hkey.n, L.stream, stride, max_tiles);
         if (v == 2 && has_bidx[bg])
             return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg], bidx[bg].desc,
                                       L.d_keys.p + bioff[bg], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
@@ -364,7 +380,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // variants launch i can run (a cached choice for the same signature may
     // name an index the table does not qualify for: then the default runs)
     auto avail = [&](size_t i, int v) {
-        if (i >= ktab.size()) return v == 0 || v == 1 || (v == 2 && has_bidx[i - ktab.size()]);
+        if (i >= ktab.size())
+            return v == 0 || v == 1 || (v == 2 && has_bidx[i - ktab.size()]) || (v == 3 && has_bslice[i - ktab.size()]);
         switch (v) {
             case 0: case 1: return true;
             case 2: return (bool)has_seed[i];
@@ -379,8 +396,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         for (size_t i = 0; i < nlaunch; i++)
             variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : 1;
     } else if (tuner && !(tuner->enabled && L.trials)) {
-        for (size_t i = 0; i < nlaunch; i++)  // default when untuned
-            if (!tuner->get(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
+        for (size_t i = 0; i < nlaunch; i++)  // the nearest known decision, else the default
+            if (!tuner->get_near(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
     } else if (tuner) {
         // time every variant of each new signature on tiles spread over the
         // genome; keep the default unless another is more than 3% faster

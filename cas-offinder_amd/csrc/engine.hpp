@@ -89,8 +89,10 @@ hipError_t launch_comparer(const DevGenome &g, uint32_t P, int nw, bool gen, uin
 // concrete tables of at most 256 keys); bulge 0 = queued exact counts, 1 = in
 // place, 2 = behind the bulge seed index (launch_bulge_index; single-class
 // A/C/G/T tables of at most 128 guides).  tile_stride / max_tiles sample
-// tiles (autotuner trials); 1 / 0 scan the whole genome.
-constexpr int kFusedVariants = 5, kBulgeVariants = 3;
+// tiles (autotuner trials); 1 / 0 scan the whole genome.  Bulge variant 3
+// runs behind the bit-sliced guide filter (launch_bulge_slice; single-class
+// A/C/G/T tables of at most 256 guides, D + R <= 2).
+constexpr int kFusedVariants = 5, kBulgeVariants = 4;
 // One-time kernel attributes (dynamic LDS limits); called when a lane opens.
 hipError_t prepare_kernels();
 hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
@@ -112,6 +114,11 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
 hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
                               const uint32_t *gap, bool rev, const uint32_t *table, const MaskDesc &md,
                               const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                              unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
+                              uint32_t max_tiles);
+hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
+                              const uint32_t *gap, bool rev, const uint32_t *table, const SliceDesc &sd,
+                              const uint32_t *slices, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                               unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
                               uint32_t max_tiles);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
@@ -185,6 +192,9 @@ struct BatchPlan {
     std::vector<BulgeTable> btab;
     std::vector<MaskIndex> bidx;  // bulge seed indexes (has_bidx)
     std::vector<char> has_bidx;
+    std::vector<SliceIndex> bslice;  // bit-sliced guide filters (has_bslice)
+    std::vector<char> has_bslice;
+    std::vector<size_t> bsoff;
     std::vector<uint32_t> kw;
     std::vector<size_t> koff, soff, boff, moff[2], bioff;
     std::vector<std::string> sig;
@@ -231,8 +241,12 @@ constexpr size_t kCandWords = (size_t)kCountSlots * kSlotStride;  // per launch
 
 // Autotuner of the batched kernels: the variant of each launch signature
 // (kernel, word count, IUPAC, alignments, pattern, table size bucket, largest
-// threshold) is chosen once per context by timing every variant on a sample
-// of tiles spread over the genome.  Shared by the lanes of a context.
+// threshold) is chosen by timing every variant on a sample of tiles spread
+// over the genome, the first time a context meets the signature, unless a
+// decision for it was loaded from a tuner file (load / save).  Lanes that may
+// not run trials (serving) take the decision of the nearest known signature:
+// same fields but the last two, nearest size bucket, then nearest threshold.
+// Shared by the lanes of a context.
 struct Tuner {
     std::mutex mu;
     std::map<std::string, int> best;
@@ -249,10 +263,17 @@ struct Tuner {
         v = it->second;
         return true;
     }
+    // exact decision, else the nearest known signature's; false when none shares the prefix
+    bool get_near(const std::string &sig, int &v);
     void set(const std::string &sig, int v) {
         std::lock_guard<std::mutex> lk(mu);
         best[sig] = v;
     }
+    // Tuner file: a JSON object {"<signature>": variant, ...}.  load merges
+    // its decisions (COF_E_IO unreadable, COF_E_INVALID malformed, err set);
+    // save writes every decision.
+    int load(const std::string &path, std::string &err);
+    int save(const std::string &path, std::string &err);
 };
 
 // Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a

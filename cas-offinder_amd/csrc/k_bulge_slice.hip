@@ -1,0 +1,285 @@
+// Batched bulge groups behind a bit-sliced filter of the guides:
+// bulge_slice_kernel.
+//
+// bulge_kernel counts each (candidate, guide) pair in its own lane.  Here a
+// lane holds one candidate and a 32-bit word holds 32 guides: per care
+// position k of the guides (one contiguous run, kn <= 32) the word of the 32
+// guides' mismatches against the candidate at window alignment u is
+//   M_u[k] = (GL[k] ^ WL[k + d_u]) | (GH[k] ^ WH[k + d_u])
+// with GL / GH the guides' code bit planes at k (one dword each, scalar
+// loads) and WL / WH the candidate's code bits at that window position
+// broadcast to all 32 bits.  A variant of the family of alignment u (DNA or
+// RNA bulge of one size, any split j) compares guide positions below j at
+// one alignment and those from j + gap at the other (cas-offinder.cpp:
+// 650-682), so its mismatches are at least popc(M_0 & M_u) - gap; the family
+// cannot hit unless the count of the planes M_0[k] & M_u[k] is at most
+// threshold + gap, and the no-bulge key (popc(M_0) <= its threshold) is
+// covered by the first family's count, which can only be smaller.  The
+// counts of 32 guides come from one carry-save adder network over the kn
+// planes (slicecount.hpp) and one bit-sliced comparison with the guides'
+// bounds.  For words of large bounds (slots are sorted by bound) each family
+// is bounded on the two halves of its split range instead, like
+// bulge_family's segments: far fewer pairs then reach the exact counts.
+// A genome N reads as base T here (no non-ACGT plane): a count can
+// only drop, so the filter stays a lower bound.  The (candidate, guide)
+// pairs that pass go to a per-wave LDS queue and are counted exactly, one
+// pair per lane, with bulge_kernel's family filters and split counts.
+#include <type_traits>
+
+#include "kbulge.hpp"
+#include "slicecount.hpp"
+
+namespace cof {
+
+struct SliceArgs {
+    BulgeArgs b;
+    SliceDesc sd;
+    const uint32_t *slices;  // word blocks, then slot records (sd.slot_off)
+};
+
+constexpr int kSliceQueue = 64;
+
+// DN / DP: window shifts -1..-DN and +1..+DP of the filter families (the
+// alignments of the table other than 0); KN: care positions of the guide
+// words (the table's kn is in (KN - 4, KN]).
+template <int DN, int DP, int KN>
+__global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
+    constexpr int NF = DN + DP, NS = NF + 1, OFF = DN > DP ? DN : DP, PN = KN + 2 * OFF;
+    constexpr int FW = PN <= 32 ? 1 : 2;  // frame words of the broadcast window positions
+    __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
+    __shared__ uint16_t s_list[kListCap];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ HitStageBuf<kStageCap> s_stage;
+    __shared__ uint32_t s_q[4][kSliceQueue];  // per wave: slot << 16 | candidate entry
+    extern __shared__ __align__(16) uint32_t s_slot[];  // nslots x kBulgeSlotWords
+    const BulgeArgs &a = sa.b;
+    const SliceDesc &sd = sa.sd;
+    for (uint32_t i = threadIdx.x; i < sd.nslots * kBulgeSlotWords / 4; i += kThreads)
+        reinterpret_cast<uint4 *>(s_slot)[i] = reinterpret_cast<const uint4 *>(sa.slices + sd.slot_off)[i];
+    // visible after tile_candidates' first barrier
+    constexpr int FB = bulge_family_words(1);
+    const kptr_t tab = (kptr_t)a.table;
+    const kptr_t blocks = (kptr_t)sa.slices;
+    const uint4 *slot4 = reinterpret_cast<const uint4 *>(s_slot);
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t *q = s_q[wave];
+    uint32_t qn = 0;  // queued pairs of this wave (uniform); the tile stays staged across its passes
+
+    // exact counts of the first n queued pairs, one per lane
+    auto drain = [&](uint64_t t0, auto rd, uint32_t n) {
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+        if (lane < n) {
+            const uint32_t item = q[lane];
+            const uint32_t slot = item >> 16, ce = item & 0xFFFFu;
+            uint32_t l[NS][1], h[NS][1], o[NS][1];
+            bulge_windows<1, NS>(a, rd, ce, l, h, o);
+            const uint4 s0 = slot4[2 * slot], s1 = slot4[2 * slot + 1];
+            const uint32_t bias[5] = {s0.w, s1.x, s1.y, s1.z, s1.w};
+            uint32_t M[NS];
+            int32_t fl[NS];
+            int32_t acc = 0;
+#pragma unroll
+            for (int u = 0; u < NS; u++) {
+                const uint32_t lc = l[u][0] & sd.care, hc = h[u][0] & sd.care, oc = o[u][0] & sd.care;
+                M[u] = __builtin_amdgcn_bitop3_b32(lc, s0.x, __builtin_amdgcn_bitop3_b32(hc, s0.y, oc, kXorOr),
+                                                   kXorOr);
+            }
+#pragma unroll
+            for (int u = 0; u < NS; u++) {
+                fl[u] = (int32_t)(bias[u] + (uint32_t)__popc(u == 0 ? M[0] : M[u] & M[0]));
+                acc |= fl[u];
+            }
+            if (acc < 0) {
+                const kptr_t vb = tab + s0.z;
+                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
+                                    stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
+#pragma unroll
+                for (int u = 0; u < NS; u++) {
+                    if (fl[u] < 0) {
+                        const bool lft0 = u == 0 || a.rev;
+                        const uint32_t left[1] = {lft0 ? M[0] : M[u]}, right[1] = {lft0 ? M[u] : M[0]};
+                        bulge_family<1>(tab, vb + 4 + FB * u, left, right, a.gap[u], bias[u], sink);
+                    }
+                }
+            }
+        }
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+    };
+
+    tile_candidates<1, true>(
+        a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum, stage_ref(s_stage),
+        a.hkey, a.hmm, a.count, a.cap,
+        [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
+            const uint32_t ce = e[0], ls = ce >> 1, strand = ce & 1;
+            // frame: bit p = window position fbase + p (reverse complemented
+            // for strand 1), broadcast per position
+            const int64_t bit = strand == 0 ? (int64_t)ls + sd.fbase + 32 * kHaloL
+                                            : (int64_t)ls + a.pd.P - 32 * FW - sd.fbase + 32 * kHaloL;
+            uint32_t fl[FW], fh[FW], fo[FW];
+            window_planes<FW>(rd, bit, strand, fl, fh, fo);
+            uint32_t WL[PN], WH[PN];
+#pragma unroll
+            for (int p = 0; p < PN; p++) {
+                WL[p] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)fl[p / 32], p % 32, 1);
+                WH[p] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)fh[p / 32], p % 32, 1);
+            }
+            const uint32_t live = valid[0] ? ~0u : 0u;
+            for (uint32_t w = 0; w < sd.nwords; w++) {
+                const kptr_t blk = blocks + w * sd.wstride;
+                uint32_t M0[KN];
+#pragma unroll
+                for (int k = 0; k < KN; k++) {
+                    M0[k] = __builtin_amdgcn_bitop3_b32(blk[k], WL[k + OFF], blk[KN + k] ^ WH[k + OFF], kXorOr);
+                    if (k >= KN - 3 && (uint32_t)k >= sd.kn) M0[k] = 0;  // positions past the care run
+                }
+                uint32_t pass = 0;
+                const bool fine = (blk[2 * KN + (kSliceTPlanes + 1) * NF] & 1u) != 0;  // uniform
+#pragma unroll
+                for (int f = 0; f < NF; f++) {
+                    const int d = f < DN ? -(f + 1) : f - DN + 1;
+                    const kptr_t tp = blk + 2 * KN + (kSliceTPlanes + 1) * f;
+                    const uint32_t t[6] = {tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
+                    // guides' mismatches at alignment u, care position k
+                    auto mu = [&](int k) {
+                        return __builtin_amdgcn_bitop3_b32(blk[k], WL[k + OFF + d], blk[KN + k] ^ WH[k + OFF + d],
+                                                           kXorOr);
+                    };
+                    uint32_t le;
+                    if (!fine) {
+                        // every split: mismatches at both alignments, at most bound + gap of them
+                        uint32_t b[KN], c[6];
+#pragma unroll
+                        for (int k = 0; k < KN; k++) b[k] = M0[k] & mu(k);
+                        slice_count<KN>(b, c);
+                        le = slice_le(c, t);
+                    } else {
+                        // splits up to care index S - 1, then splits from S: below the
+                        // second half's splits only the left part compares, from S + 1
+                        // (the largest gap is 2) on only the right part of the first
+                        // half's.  The left part is alignment u for a forward PAM and
+                        // alignment 0 for a reversed one.  Each half's count is the sum
+                        // of the counts over care positions [0, S) and [S, KN), so that
+                        // only half-width planes are live at once.
+                        constexpr int S = KN / 2;
+                        auto halves = [&](auto rvc) {
+                            constexpr bool RV = decltype(rvc)::value;
+                            uint32_t lo1[S], lo2[S], a1[6], a2[6], b1[6], b2[6], h1[6], h2[6];
+#pragma unroll
+                            for (int k = 0; k < S; k++) {
+                                const uint32_t m = mu(k);
+                                lo1[k] = M0[k] & m;
+                                lo2[k] = RV ? M0[k] : m;
+                            }
+                            slice_count<S>(lo1, a1);
+                            slice_count<S>(lo2, a2);
+                            uint32_t hi1[KN - S], hi2[KN - S];
+#pragma unroll
+                            for (int k = S; k < KN; k++) {
+                                const uint32_t m = mu(k), b = M0[k] & m;
+                                hi1[k - S] = k == S ? b : RV ? m : M0[k];
+                                hi2[k - S] = b;
+                            }
+                            slice_count<KN - S>(hi1, b1);
+                            slice_count<KN - S>(hi2, b2);
+                            slice_add(a1, b1, h1);
+                            slice_add(a2, b2, h2);
+                            return slice_le(h1, t) | slice_le(h2, t);
+                        };
+                        le = a.rev ? halves(std::true_type{}) : halves(std::false_type{});
+                    }
+                    pass |= le & tp[kSliceTPlanes];
+                }
+                pass &= live;
+                // queue the passing (candidate, guide) pairs, one per lane per round
+                for (;;) {
+                    const uint64_t m = __ballot(pass != 0);
+                    if (!m) break;
+                    const uint32_t np = (uint32_t)__popcll(m);
+                    if (qn + np > (uint32_t)kSliceQueue) {
+                        drain(t0, rd, qn);
+                        qn = 0;
+                    }
+                    if (pass) {
+                        const uint32_t g = (uint32_t)__builtin_ctz(pass);
+                        pass &= pass - 1;
+                        const uint32_t at =
+                            qn + __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+                        q[at] = ((32 * w + g) << 16) | ce;
+                    }
+                    qn += np;
+                }
+            }
+        },
+        0u, nullptr,
+        [&](uint64_t t0, auto rd) {
+            if (qn) drain(t0, rd, qn);
+            qn = 0;
+        });
+}
+
+template <int DN, int DP, int KN>
+static hipError_t slice_launch(dim3 grid, const SliceArgs &a, uint32_t lds, hipStream_t s) {
+    note_kernel("bulge_slice_kernel", DN, DP, KN);
+    hipLaunchKernelGGL((bulge_slice_kernel<DN, DP, KN>), grid, dim3(kThreads), lds, s, a);
+    return hipGetLastError();
+}
+
+template <int DN, int DP>
+static hipError_t slice_kn(dim3 grid, const SliceArgs &a, uint32_t lds, hipStream_t s) {
+    switch (a.sd.KN) {
+        case 16: return slice_launch<DN, DP, 16>(grid, a, lds, s);
+        case 20: return slice_launch<DN, DP, 20>(grid, a, lds, s);
+        case 24: return slice_launch<DN, DP, 24>(grid, a, lds, s);
+        case 28: return slice_launch<DN, DP, 28>(grid, a, lds, s);
+        case 32: return slice_launch<DN, DP, 32>(grid, a, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
+                              const uint32_t *gap, bool rev, const uint32_t *table, const SliceDesc &sd,
+                              const uint32_t *slices, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
+                              unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
+                              uint32_t max_tiles) {
+    const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
+    if (limit == 0 || sd.nslots == 0) return hipSuccess;
+    // the kernel reads nwords blocks of wstride dwords and nslots slot records
+    // after slot_off, 2 * KN + 7 * (dn + dp) dwords per block, and care
+    // positions kn of the KN it is built for
+    if (tile_stride == 0 || !slice_desc_valid(sd, ns) || (uintptr_t)slices % 16) return hipErrorInvalidValue;
+    SliceArgs a{};
+    BulgeArgs &b = a.b;
+    b.g = g;
+    b.pd = pd;
+    for (int u = 0; u < ns; u++) {
+        b.cb[u] = cb[u];
+        b.gap[u] = gap[u];
+    }
+    b.rev = rev ? 1u : 0u;
+    b.nunits = 1;
+    b.tile_stride = tile_stride;
+    b.table = table;
+    b.limit = limit;
+    b.hkey = hkey;
+    b.hmm = hmm;
+    b.count = count;
+    b.cand_count = cand_count;
+    b.cap = cap;
+    a.sd = sd;
+    a.slices = slices;
+    const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
+    const uint32_t lds = sd.nslots * kBulgeSlotWords * 4;
+    switch (sd.dn * 3 + sd.dp) {
+        case 1: return slice_kn<0, 1>(grid, a, lds, s);
+        case 2: return slice_kn<0, 2>(grid, a, lds, s);
+        case 3: return slice_kn<1, 0>(grid, a, lds, s);
+        case 4: return slice_kn<1, 1>(grid, a, lds, s);
+        case 6: return slice_kn<2, 0>(grid, a, lds, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace cof

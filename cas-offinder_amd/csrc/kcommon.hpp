@@ -551,13 +551,19 @@ constexpr int kListCap = 4096;  // LDS candidate entries per pass
 // (the pointers passed are the group's); the groups run the same number of
 // candidate passes, so that every barrier is reached by all of them.
 // s_nmax: G words of LDS (G > 1 only).
-template <int W, bool ALL = false, int LC = kListCap, int G = 1, typename Cands>
+// fin(t0, rd) runs on every thread after the candidates of each pass and
+// before the stage is flushed (per-wave queues drain there).
+struct NoFin {
+    template <typename Rd>
+    __device__ void operator()(uint64_t, Rd) const {}
+};
+template <int W, bool ALL = false, int LC = kListCap, int G = 1, typename Cands, typename Fin = NoFin>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 uint32_t tile_stride, unsigned long long *cand_count,
                                                 uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot, uint16_t *s_list,
                                                 uint32_t *s_wsum, StageRef st, uint64_t *hkey, uint16_t *hmm,
                                                 unsigned long long *count, uint64_t cap, Cands &&cands,
-                                                uint32_t ntile = 0, uint32_t *s_nmax = nullptr) {
+                                                uint32_t ntile = 0, uint32_t *s_nmax = nullptr, Fin &&fin = Fin{}) {
     const int t = G == 1 ? (int)threadIdx.x : (int)(threadIdx.x & (kThreads - 1));
     const uint32_t grp = G == 1 ? 0u : threadIdx.x / kThreads;
     const uint32_t gi = G == 1 ? blockIdx.x : blockIdx.x * G + grp;
@@ -638,6 +644,7 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
                 cands(t0, rd, e, valid);
             }
         }
+        fin(t0, rd);
         __syncthreads();
         flush_stage(st, t0, hkey, hmm, count, cap, t);
     }

@@ -3,6 +3,7 @@
 #include "query.hpp"
 
 #include <algorithm>
+#include <array>
 #include <cstdlib>
 #include <cstring>
 #include <map>
@@ -688,6 +689,102 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
             for (int u = 0; u < ns; u++) t.tmax = std::max<int>(t.tmax, bg.tmax[u]);
     }
     return 1;
+}
+
+bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &s) {
+    s = SliceIndex();
+    if (t.gen || t.nw != 1 || t.units != 1 || t.words.size() < (size_t)kClassHdr) return false;
+    const int ns = t.ns, KP = bulge_planes(1, false), S = bulge_stride(1, false, ns);
+    const uint32_t care = t.words[0], n = t.words[1];
+    if (n == 0 || n > kSliceMaxGuides || care == 0) return false;
+    const int k0 = __builtin_ctz(care), kn = __builtin_popcount(care);
+    if ((care >> k0) != (kn == 32 ? ~0u : (1u << kn) - 1)) return false;  // one run of care positions
+    // filter families: window shift of each alignment relative to alignment 0
+    SliceDesc &d = s.desc;
+    int u_of[2 * 2] = {-1, -1, -1, -1};  // family f -> table shift u
+    int dn = 0, dp = 0;
+    for (int u = 1; u < ns; u++) {
+        const int delta = t.cb[u] - t.cb[0];
+        if (delta < -2 || delta > 2 || delta == 0) return false;
+        if (delta < 0) dn = std::max(dn, -delta);
+        else dp = std::max(dp, delta);
+    }
+    if (dn + dp != ns - 1 || ns > 3) return false;  // one alignment per shift, at most two families
+    for (int u = 1; u < ns; u++) {
+        const int delta = t.cb[u] - t.cb[0];
+        const int f = delta < 0 ? -delta - 1 : dn + delta - 1;
+        if (u_of[f] >= 0) return false;
+        u_of[f] = u;
+    }
+    const int nf = dn + dp, off = std::max(dn, dp);
+    d.KN = kn <= 16 ? 16 : kn <= 20 ? 20 : kn <= 24 ? 24 : kn <= 28 ? 28 : 32;
+    d.kn = (uint32_t)kn;
+    d.dn = (uint32_t)dn;
+    d.dp = (uint32_t)dp;
+    d.fbase = t.cb[0] + k0 - off;
+    d.care = care;
+    d.nslots = n;
+    d.nwords = (n + 31) / 32;
+    // every LDS word the kernel reads for a window start in the tile stays
+    // inside the staged tile + halo, on both strands (frame of fw words)
+    const int fw = (int)d.KN + 2 * off <= 32 ? 1 : 2;
+    {
+        const int64_t P = pd.P, halo = 32 * kHaloL;
+        const int64_t plus_lo = d.fbase + halo, plus_hi = kTilePos - 1 + d.fbase + halo + 32 * (fw - 1);
+        const int64_t minus_lo = P - 32 * fw - d.fbase + halo;
+        const int64_t minus_hi = kTilePos - 1 + P - 32 * fw - d.fbase + halo + 32 * (fw - 1);
+        if (plus_lo < 0 || minus_lo < 0 || (plus_hi >> 5) + 1 >= kLdsWords || (minus_hi >> 5) + 1 >= kLdsWords)
+            return false;
+    }
+    d.wstride = (uint32_t)((2 * d.KN + (kSliceTPlanes + 1) * nf + 1 + 3) & ~3u);
+    d.slot_off = d.nwords * d.wstride;
+    if (!slice_desc_valid(d, ns)) return false;  // e.g. care runs shorter than 13 positions
+    // count bound of each filter family per guide record (-1: the family
+    // never emits); the no-bulge key's threshold joins family 0
+    std::vector<std::array<int, 4>> bound(n);
+    for (uint32_t r = 0; r < n; r++) {
+        const uint32_t *rec = &t.words[kClassHdr + (size_t)r * S];
+        auto tof = [&](int u) {
+            const uint32_t b = rec[KP + 1 + u];
+            return b == kNeverHit ? -1 : (int)(-(int32_t)b - 1);
+        };
+        for (int f = 0; f < nf; f++) {
+            int T = tof(u_of[f]);
+            if (f == 0) T = std::max(T, tof(0));
+            bound[r][f] = std::min(T, (1 << kSliceTPlanes) - 1);
+        }
+    }
+    auto top = [&](uint32_t r) { return *std::max_element(bound[r].begin(), bound[r].begin() + nf); };
+    std::vector<uint32_t> order(n);
+    for (uint32_t r = 0; r < n; r++) order[r] = r;
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return top(x) > top(y); });
+    s.words.assign(d.slot_off + (size_t)kBulgeSlotWords * n, 0);
+    for (uint32_t si = 0; si < n; si++) {  // slot si holds guide record order[si]
+        const uint32_t r = order[si];
+        const uint32_t *rec = &t.words[kClassHdr + (size_t)r * S];
+        uint32_t *blk = &s.words[(size_t)(si / 32) * d.wstride];
+        const uint32_t bit = 1u << (si % 32);
+        for (int k = 0; k < kn; k++) {
+            if ((rec[0] >> (k0 + k)) & 1) blk[k] |= bit;
+            if ((rec[1] >> (k0 + k)) & 1) blk[d.KN + k] |= bit;
+        }
+        for (int f = 0; f < nf; f++) {
+            const int T = bound[r][f];
+            if (T < 0) continue;
+            uint32_t *fp = blk + 2 * d.KN + (kSliceTPlanes + 1) * f;
+            for (int p = 0; p < kSliceTPlanes; p++)
+                if ((T >> p) & 1) fp[p] |= bit;
+            fp[kSliceTPlanes] |= bit;
+        }
+        if (top(r) >= kSliceFineT) blk[2 * d.KN + (kSliceTPlanes + 1) * nf] |= 1u;
+        uint32_t *slot = &s.words[d.slot_off + (size_t)kBulgeSlotWords * si];
+        slot[0] = rec[0];
+        slot[1] = rec[1];
+        slot[2] = rec[KP];
+        for (int u = 0; u < kMaxShifts; u++) slot[3 + u] = u < ns ? rec[KP + 1 + u] : kNeverHit;
+    }
+    s.words.resize((s.words.size() + 3) & ~(size_t)3, 0);
+    return true;
 }
 
 bool build_bulge_index(const BulgeTable &t, MaskIndex &s) {

@@ -152,6 +152,24 @@ struct BulgeTable {
 // or thresholds too high for the comparable positions.
 bool build_bulge_index(const BulgeTable &t, MaskIndex &s);
 
+// Bit-sliced filter tables of a bulge table (bulge_slice_kernel, SliceDesc
+// layout).  A variant of family u is within its threshold only if the
+// guide's mismatches at alignment 0 and at alignment u, both counted over
+// the care run, have at most (threshold + RNA gap) positions in common
+// (cas-offinder.cpp:650-682: the left part of a split compares at one
+// alignment, the right part at the other), and the no-bulge key only if
+// its own count is at most its threshold, which bounds the first family's
+// count too; so slot g passes family f when popc(M0 & Mf) <= T_f(g).  False
+// for IUPAC guides, several care classes, a care run with holes or longer
+// than 32, two-word windows, more than two bulge families (D + R > 2) or
+// bulges above 2, more than
+// kSliceMaxGuides guides or windows outside the staged tile.
+struct SliceIndex {
+    SliceDesc desc{};
+    std::vector<uint32_t> words;
+};
+bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &s);
+
 struct BulgeRef {
     const Query *q;
     uint32_t gbase;  // global index of the query's first key

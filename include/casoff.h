@@ -167,6 +167,15 @@ int cof_stats_get(const cof_ctx *ctx, cof_stats *out);
  * each (no reference counterpart: the reference's launch shape is fixed).
  * Caller frees *out with cof_free. */
 int cof_tuner_info(cof_ctx *ctx, char **out, size_t *len);
+/* Tuner file (no reference counterpart): a JSON object {"signature":
+ * variant, ...} of autotuner decisions.  A context opens with the decisions
+ * of $COF_TUNER_FILE ("none": no file), else of tuned_gfx950.json beside
+ * libcasoff.so when it exists (cof_open fails with COF_E_INVALID when that
+ * file is malformed); launches whose signature it holds run no trials.
+ * cof_tuner_load merges a file's decisions into the context's; cof_tuner_save
+ * writes all of them. */
+int cof_tuner_load(cof_ctx *ctx, const char *path);
+int cof_tuner_save(cof_ctx *ctx, const char *path);
 
 /* Batched launches of the last cof_search on the context, one line each:
  * "kernel<TAB>signature<TAB>variant<TAB>ms<TAB>candidates": the kernel
