@@ -113,6 +113,7 @@ def time_steps(ctx, qs, mode, steps, warmup):
         tune_ms += ctx.stats()["t_tune_ms"]
     time_steps.tune_ms = tune_ms
     kernel_ms, launches, per_kernel, collect = 0.0, 0, {}, 0.0
+    parts = {"sort_ms": 0.0, "copy_ms": 0.0, "decode_ms": 0.0}
     t0 = time.perf_counter()
     for _ in range(steps):
         r = ctx.search(qs, mode)
@@ -120,6 +121,8 @@ def time_steps(ctx, qs, mode, steps, warmup):
         kernel_ms += st["t_fused_ms"] if mode == casoff.MODE_BATCHED else st["t_finder_ms"] + st["t_comparer_ms"]
         launches += st["n_launches"]
         collect += st["t_collect_ms"]
+        for k in parts:
+            parts[k] += st["t_" + k]
         if mode == casoff.MODE_BATCHED:
             for rec in ctx.launch_info():
                 k = per_kernel.setdefault(rec["kernel"], {"ms": 0.0, "launches": 0})
@@ -127,6 +130,7 @@ def time_steps(ctx, qs, mode, steps, warmup):
                 k["launches"] += 1
     t1 = time.perf_counter()
     time_steps.collect_ms = collect / steps
+    time_steps.collect_parts = {k: round(v / steps, 3) for k, v in parts.items()}
     return (t1 - t0) / steps, kernel_ms / steps, launches / steps, st, r, per_kernel
 
 
@@ -167,6 +171,7 @@ def run_workload(ctx, name, steps, warmup, plain_steps, with_plain=True):
     log("batched: %.2f ms/step (kernels %.2f ms), %d hits" % (s_b * 1e3, k_b, len(hits_b)))
     res = {"value": round(n_guides * total / 1e9 / s_b, 2), "ms_per_step": round(s_b * 1e3, 3), "guides": n_guides,
            "queries": len(qs), "kernel_ms_per_step": round(k_b, 3), "collect_ms_per_step": round(time_steps.collect_ms, 3),
+           "collect_parts": time_steps.collect_parts,
            "launches_per_step": l_b, "hits": len(hits_b),
            "candidate_sites_per_s": st_b["n_pairs"] / s_b,
            "counters": {"candidates": st_b["n_candidates"], "pairs": st_b["n_pairs"], "hits": len(hits_b),

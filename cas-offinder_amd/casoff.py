@@ -53,6 +53,9 @@ class Stats(C.Structure):
         ("n_finder_positions", C.c_uint64),
         ("t_tune_ms", C.c_double),
         ("graph", C.c_uint64),
+        ("t_sort_ms", C.c_double),
+        ("t_copy_ms", C.c_double),
+        ("t_decode_ms", C.c_double),
     ]
 
     def as_dict(self):

@@ -115,6 +115,7 @@ int cof_open(int device, cof_ctx **out) {
     if (const char *at = getenv("COF_AUTOTUNE")) c->tuner.enabled = strcmp(at, "0") != 0;
     if (const char *fv = getenv("COF_FORCE_VARIANT")) c->tuner.force = std::min(std::max(atoi(fv), 0), cof::kFusedVariants - 1);
     if (const char *gr = getenv("COF_GRAPH")) c->lane.graphs = strcmp(gr, "0") != 0;
+    if (const char *hp = getenv("COF_HIT_PACK")) c->lane.pack_hits = strcmp(hp, "0") != 0;
     if (!tf.empty() && c->tuner.load(tf, g_noctx_err)) {
         delete c;
         return COF_E_INVALID;

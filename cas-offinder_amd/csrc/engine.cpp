@@ -569,6 +569,14 @@ void Lane::release() {
     d_sort_tmp.release();
     h_hkey.release();
     h_hmm.release();
+    d_pos32.release();
+    d_koff.release();
+    d_ms.release();
+    h_pos32.release();
+    h_koff.release();
+    h_ms.release();
+    for (hipEvent_t x : cev) (void)hipEventDestroy(x);
+    cev.clear();
     d_count.release();
     d_loci.release();
     d_flags.release();
@@ -618,6 +626,7 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     }
     if (nhits > L.d_hkey.n) { err = "Hit buffer overflow."; return COF_E_NOMEM; }
     const auto t_kern = std::chrono::steady_clock::now();
+    HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
     // sort on the device by (global key, start, strand), copy to pinned memory
     const uint64_t *hk = L.d_hkey.p;
     const uint16_t *hm = L.d_hmm.p;
@@ -637,31 +646,96 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
         hk = L.d_hkey2.p;
         hm = L.d_hmm2.p;
     }
-    HIP_TRY(L.h_hkey.reserve(nhits), "hipHostMalloc(hits)");
-    HIP_TRY(L.h_hmm.reserve(nhits), "hipHostMalloc(hits)");
-    if (nhits) {
-        HIP_TRY(hipMemcpyAsync(L.h_hkey.p, hk, nhits * 8, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
-        HIP_TRY(hipMemcpyAsync(L.h_hmm.p, hm, nhits * 2, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
-    }
-    HIP_TRY(hipStreamSynchronize(L.stream), "hit copy");
     auto *r = new cof_result();
     r->hits.resize(nhits);
-    const uint64_t *hkp = L.h_hkey.p;
-    const uint16_t *hmp = L.h_hmm.p;
-    parallel_for(nhits, 1 << 16, [&](uint64_t b, uint64_t e) {
-        size_t q = 0;
-        for (uint64_t i = b; i < e; i++) {
-            const uint64_t k = hkp[i];
-            const uint32_t gk = (uint32_t)(k >> kHitKeyShift);
-            while (q + 1 < qs.size() && gk >= gbase[q + 1]) q++;  // keys are sorted
-            cof_hit &h = r->hits[i];
-            h.pos = (k >> 1) & (kMaxGenomePositions - 1);
-            h.strand = (uint8_t)(k & 1);
-            h.key = gk - gbase[q];
-            h.mm = hmp[i];
-            h.query = (uint8_t)q;
+    double t_decode = 0;
+    if (nhits && L.pack_hits && dg.total < (1ull << 32) && nhits < (1ull << 32)) {
+        // packed: 5 bytes per hit and the first hit of each key cross PCIe,
+        // in chunks; each chunk is decoded by host threads while the next
+        // one is copied
+        const uint64_t nk = nkeys + 1;
+        HIP_TRY(L.d_pos32.reserve(nhits), "hipMalloc(pack)");
+        HIP_TRY(L.d_ms.reserve(nhits), "hipMalloc(pack)");
+        HIP_TRY(L.d_koff.reserve(nk), "hipMalloc(pack)");
+        HIP_TRY(L.h_pos32.reserve(nhits), "hipHostMalloc(pack)");
+        HIP_TRY(L.h_ms.reserve(nhits), "hipHostMalloc(pack)");
+        HIP_TRY(L.h_koff.reserve(nk), "hipHostMalloc(pack)");
+        HIP_TRY(pack_hits(hk, hm, nhits, (uint32_t)nkeys, L.d_pos32.p, L.d_ms.p, L.d_koff.p, L.stream), "pack hits");
+        HIP_TRY(hipEventRecord(L.ev[3], L.stream), "hipEventRecord");
+        HIP_TRY(hipMemcpyAsync(L.h_koff.p, L.d_koff.p, nk * 4, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(pack)");
+        constexpr uint64_t kChunk = 1ull << 22;
+        const uint64_t nch = (nhits + kChunk - 1) / kChunk;
+        while (L.cev.size() < nch) {
+            hipEvent_t e = nullptr;
+            HIP_TRY(hipEventCreate(&e), "hipEventCreate");
+            L.cev.push_back(e);
         }
-    });
+        for (uint64_t c = 0; c < nch; c++) {
+            const uint64_t b = c * kChunk, n = std::min(nhits, b + kChunk) - b;
+            HIP_TRY(hipMemcpyAsync(L.h_pos32.p + b, L.d_pos32.p + b, n * 4, hipMemcpyDeviceToHost, L.stream),
+                    "hipMemcpy(pack)");
+            HIP_TRY(hipMemcpyAsync(L.h_ms.p + b, L.d_ms.p + b, n, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(pack)");
+            HIP_TRY(hipEventRecord(L.cev[c], L.stream), "hipEventRecord");
+        }
+        HIP_TRY(hipEventRecord(L.ev[4], L.stream), "hipEventRecord");
+        const uint32_t *koff = L.h_koff.p, *pos = L.h_pos32.p;
+        const uint8_t *ms = L.h_ms.p;
+        for (uint64_t c = 0; c < nch; c++) {
+            HIP_TRY(hipEventSynchronize(L.cev[c]), "hit copy");
+            const auto td = std::chrono::steady_clock::now();
+            const uint64_t b0 = c * kChunk, e0 = std::min(nhits, b0 + kChunk);
+            parallel_for(e0 - b0, 1 << 16, [&](uint64_t b, uint64_t e) {
+                b += b0, e += b0;
+                // the key of hit b: the last key whose first hit is at or before b
+                uint64_t k = (uint64_t)(std::upper_bound(koff, koff + nk, (uint32_t)b) - koff) - 1;
+                size_t q = (size_t)(std::upper_bound(gbase.begin(), gbase.end() - 1, (uint32_t)k) - gbase.begin()) - 1;
+                for (uint64_t i = b; i < e; i++) {
+                    while (koff[k + 1] <= i) k++;
+                    while (q + 1 < qs.size() && k >= gbase[q + 1]) q++;
+                    cof_hit &h = r->hits[i];
+                    h.pos = pos[i];
+                    h.key = (uint32_t)k - gbase[q];
+                    h.mm = (uint16_t)(ms[i] >> 1);
+                    h.strand = (uint8_t)(ms[i] & 1);
+                    h.query = (uint8_t)q;
+                }
+            });
+            t_decode += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td).count();
+        }
+        L.stats.t_sort_ms = ms_between(L.ev[2], L.ev[3]);
+        L.stats.t_copy_ms = ms_between(L.ev[3], L.ev[4]);
+    } else {
+        HIP_TRY(L.h_hkey.reserve(nhits), "hipHostMalloc(hits)");
+        HIP_TRY(L.h_hmm.reserve(nhits), "hipHostMalloc(hits)");
+        HIP_TRY(hipEventRecord(L.ev[3], L.stream), "hipEventRecord");
+        if (nhits) {
+            HIP_TRY(hipMemcpyAsync(L.h_hkey.p, hk, nhits * 8, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
+            HIP_TRY(hipMemcpyAsync(L.h_hmm.p, hm, nhits * 2, hipMemcpyDeviceToHost, L.stream), "hipMemcpy(hits)");
+        }
+        HIP_TRY(hipEventRecord(L.ev[4], L.stream), "hipEventRecord");
+        HIP_TRY(hipStreamSynchronize(L.stream), "hit copy");
+        const auto td = std::chrono::steady_clock::now();
+        const uint64_t *hkp = L.h_hkey.p;
+        const uint16_t *hmp = L.h_hmm.p;
+        parallel_for(nhits, 1 << 16, [&](uint64_t b, uint64_t e) {
+            size_t q = 0;
+            for (uint64_t i = b; i < e; i++) {
+                const uint64_t k = hkp[i];
+                const uint32_t gk = (uint32_t)(k >> kHitKeyShift);
+                while (q + 1 < qs.size() && gk >= gbase[q + 1]) q++;  // keys are sorted
+                cof_hit &h = r->hits[i];
+                h.pos = (k >> 1) & (kMaxGenomePositions - 1);
+                h.strand = (uint8_t)(k & 1);
+                h.key = gk - gbase[q];
+                h.mm = hmp[i];
+                h.query = (uint8_t)q;
+            }
+        });
+        t_decode = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td).count();
+        L.stats.t_sort_ms = ms_between(L.ev[2], L.ev[3]);
+        L.stats.t_copy_ms = ms_between(L.ev[3], L.ev[4]);
+    }
+    L.stats.t_decode_ms = t_decode;
     r->queries = qs;
     r->qbegin.assign(qs.size() + 1, 0);
     for (size_t qi = 0; qi <= qs.size(); qi++)  // hits are sorted by query

@@ -123,6 +123,11 @@ hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns,
                               uint32_t max_tiles);
 hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t *kout, const uint16_t *vin,
                      uint16_t *vout, uint64_t n, unsigned end_bit, hipStream_t s);
+// Packs n sorted hits (genome below 2^32 positions) into pos[i] (window
+// start) and ms[i] (mismatches << 1 | strand), and koff[0..nkeys] into the
+// first hit of each global key (nkeys: the search's key count).
+hipError_t pack_hits(const uint64_t *hkey, const uint16_t *hmm, uint64_t n, uint32_t nkeys, uint32_t *pos, uint8_t *ms,
+                     uint32_t *koff, hipStream_t s);
 // Kernel instance of the calling thread's last batched launch, as profilers
 // name it ("cof::fused_mask_kernel<20, 3, 8, 2>"); empty after a reset when
 // the launcher had nothing to launch.
@@ -209,6 +214,12 @@ struct Lane {
     DevBuf<uint8_t> d_sort_tmp;
     HostBuf<uint64_t> h_hkey;
     HostBuf<uint16_t> h_hmm;
+    // packed hits for the copy to the host (pack_hits)
+    DevBuf<uint32_t> d_pos32, d_koff;
+    DevBuf<uint8_t> d_ms;
+    HostBuf<uint32_t> h_pos32, h_koff;
+    HostBuf<uint8_t> h_ms;
+    std::vector<hipEvent_t> cev;  // copy chunks
     // [0] hits; from kCandBase, per batched launch (or for the plain finder),
     // kCountSlots candidate-count slots kSlotStride words apart
     DevBuf<unsigned long long> d_count;
@@ -231,6 +242,7 @@ struct Lane {
     void drop_graphs();
     bool graphs = true;
     bool trials = true;  // may run autotuner trials (off where latency matters: serving)
+    bool pack_hits = true;  // packed hit copies where the genome allows (COF_HIT_PACK=0: 10-byte records)
     std::shared_ptr<BatchPlan> plan;  // of the last batched search
     hipError_t init();
     void release();

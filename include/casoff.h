@@ -69,8 +69,11 @@ typedef struct {
     uint64_t n_finder_positions; /* plain mode: window starts the finder launches scanned */
     double t_tune_ms;       /* batched mode: autotuner trials run by this search (first use of a
                                launch signature on the context; COF_AUTOTUNE=0 disables them) */
-    uint64_t graph;         /* batched mode: 0 launches issued one by one, 1 captured as a HIP
-                               graph and launched, 2 the previous search's graph replayed */
+    uint64_t graph;         /* batched mode: 0 launches issued one by one, 1 captured as HIP
+                               graphs and launched, 2 the previous search's graphs replayed */
+    double t_sort_ms;       /* collection: device sort and packing of the hits (HIP events) */
+    double t_copy_ms;       /* collection: device-to-host copies of the hits (HIP events) */
+    double t_decode_ms;     /* collection: host decoding into cof_hit records (overlaps the copies) */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference

@@ -118,3 +118,37 @@ def test_plain_search_between_batched_searches(tmp_path):
         assert modes[5] == (casoff.MODE_BATCHED, 2), modes
     finally:
         ctx.close()
+
+
+def test_hit_copy_forms_agree(tmp_path):
+    """Hits cross to the host packed (5 bytes and a per-key first-hit table)
+    below 2^32 genome positions, and as 10-byte records otherwise
+    (COF_HIT_PACK=0 forces that form here): both decode to the same records,
+    over several copy chunks (a workload of more than 4 M hits)."""
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=24, n_contigs=3, total_len=12_000_000, n_frac=0.01), gp, 0)
+    # all-N pattern, 8 comparable positions at threshold 3: ~6% of the windows
+    # hit each guide, on both strands (~6 M hits)
+    guides = ["ACGTACGT", "TTGCACGT", "CCGTACGA", "GATTACAG", "AACCGGTT", "TGCATGCA", "CATGACTA", "GGACTTCA"]
+    text = gp + "\nNNNNNNNNNNNNNNNN\n" + "".join("%sNNNNNNNN 3\n" % g for g in guides)
+    got = {}
+    for pack in ("1", "0"):
+        old = os.environ.get("COF_HIT_PACK")
+        os.environ["COF_HIT_PACK"] = pack
+        try:
+            c = casoff.Context(0)
+        finally:
+            if old is None:
+                os.environ.pop("COF_HIT_PACK")
+            else:
+                os.environ["COF_HIT_PACK"] = old
+        try:
+            c.load(gp)
+            q = c.query(text)
+            got[pack] = [c.search([q], m).records() for m in (casoff.MODE_BATCHED, casoff.MODE_PLAIN)]
+        finally:
+            c.close()
+    assert len(got["1"][0]) > (1 << 22) + 1000
+    for a, b in zip(got["1"], got["0"]):
+        assert len(a) == len(b) and (a == b).all()
+    assert (got["1"][0] == got["1"][1]).all()
