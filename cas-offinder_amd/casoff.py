@@ -100,6 +100,7 @@ def lib():
             "cof_query_info": (i32, [vp, C.POINTER(C.c_char_p), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
             "cof_search": (i32, [vp, C.POINTER(vp), i32, i32, C.POINTER(vp)]),
             "cof_stats_get": (i32, [vp, C.POINTER(Stats)]),
+            "cof_search_noalloc": (i32, [vp, C.POINTER(vp), i32, vp, vp, u64, C.POINTER(u64)]),
             "cof_tuner_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_launch_info": (i32, [vp, C.POINTER(C.c_void_p), C.POINTER(sz)]),
             "cof_tuner_load": (i32, [vp, C.c_char_p]),
@@ -285,6 +286,14 @@ class Context:
         if rc:
             raise CofError(self.error())
         return Result(self, out)
+
+    def search_noalloc(self, queries, hit_keys_ptr, hit_mm_ptr, cap):
+        """cof_search_noalloc into caller-owned device buffers (raw device
+        pointers); returns (status, hits found)."""
+        arr = (C.c_void_p * len(queries))(*[q._p.value for q in queries])
+        n = C.c_uint64()
+        rc = lib().cof_search_noalloc(self._p, arr, len(queries), hit_keys_ptr, hit_mm_ptr, cap, C.byref(n))
+        return rc, n.value
 
     def stats(self):
         s = Stats()

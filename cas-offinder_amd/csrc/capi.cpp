@@ -120,6 +120,11 @@ int cof_open(int device, cof_ctx **out) {
         delete c;
         return COF_E_INVALID;
     }
+    if (noalloc_init(device) != hipSuccess) {
+        delete c;
+        g_noctx_err = "cannot look up the scratch array's device address";
+        return COF_E_DEVICE;
+    }
     if (c->lane.init() != hipSuccess) {
         c->lane.release();
         delete c;

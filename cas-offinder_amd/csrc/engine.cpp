@@ -282,7 +282,8 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
 //    compare keys go into one table (fused_kernel).
 // Records carry global key indices.
 int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
-                   const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err) {
+                   const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err,
+                   const ExtBufs *ext = nullptr) {
     // the plan of the lane's previous search is reused for the same query
     // objects (it holds them, so their addresses identify them)
     if (!(L.plan && L.plan->qs == qs && L.plan->gbase == gbase)) {
@@ -310,15 +311,40 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     const auto &bslice = P.bslice;
     const auto &has_bslice = P.has_bslice;
     const auto &bsoff = P.bsoff;
-    HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     const size_t nlaunch = ktab.size() + btab.size();
+    // device buffers: the lane's, or the caller's (no allocation, no graphs,
+    // no trials, no per-launch timing)
+    uint32_t *d_keys;
+    unsigned long long *d_count;
+    size_t count_n;
+    uint64_t *d_hkey;
+    uint16_t *d_hmm;
+    uint64_t cap;
+    if (ext) {
+        if (kw.size() > ext->keys_words) { err = "Key tables exceed the scratch buffer."; return COF_E_NOMEM; }
+        d_keys = ext->keys;
+        d_count = ext->count;
+        count_n = ext->count_words;
+        d_hkey = ext->hkey;
+        d_hmm = ext->hmm;
+        cap = ext->cap;
+    } else {
+        HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
+        d_keys = L.d_keys.p;
+        d_count = L.d_count.p;
+        count_n = L.d_count.n;
+        d_hkey = L.d_hkey.p;
+        d_hmm = L.d_hmm.p;
+        cap = L.d_hkey.n;
+    }
+    const bool graphs = L.graphs && !ext;
     // identity of the launch sequence: every kernel argument and the tables
     std::string gkey;
-    if (L.graphs) {
+    if (graphs) {
         auto put = [&](const void *p, size_t n) { gkey.append(static_cast<const char *>(p), n); };
-        const void *ptrs[] = {L.d_keys.p, L.d_hkey.p, L.d_hmm.p, L.d_count.p, dg.lo, dg.poison};
+        const void *ptrs[] = {d_keys, d_hkey, d_hmm, d_count, dg.lo, dg.poison};
         put(ptrs, sizeof ptrs);
-        put(&L.d_hkey.n, sizeof L.d_hkey.n);
+        put(&cap, sizeof cap);
         put(&dg.total, sizeof dg.total);
         for (size_t g = 0; g < ktab.size(); g++) {
             const int v[4] = {ktab[g].nw, ktab[g].gen, ktab[g].cbase, (int)ktab[g].units};
@@ -342,38 +368,38 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         }
         put(kw.data(), kw.size() * 4);
     }
-    const bool replay = !L.graph.empty() && L.graphs && gkey == L.graph_key;
+    const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key;
     if (!replay && !kw.empty())
-        HIP_TRY(hipMemcpyAsync(L.d_keys.p, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
+        HIP_TRY(hipMemcpyAsync(d_keys, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
-    if (kCandBase + nlaunch * kCandWords > L.d_count.n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
-    auto cand_slots = [&](size_t i) { return L.d_count.p + kCandBase + i * kCandWords; };
+    if (kCandBase + nlaunch * kCandWords > count_n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
+    auto cand_slots = [&](size_t i) { return d_count + kCandBase + i * kCandWords; };
     // launch i < ktab.size(): fused key group i; else bulge group i - ktab.size()
     auto launch = [&](size_t i, int v, uint32_t stride, uint32_t max_tiles) {
         if (i < ktab.size() && (v == 3 || v == 4) && has_mask[v - 3][i])
             return launch_mask(dg, krep[i]->pat, masks[v - 3][i].desc, (uint32_t)ktab[i].cbase,
-                               L.d_keys.p + moff[v - 3][i], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
-                               L.d_hkey.n, L.stream, stride, max_tiles);
+                               d_keys + moff[v - 3][i], d_hkey, d_hmm, d_count, cand_slots(i),
+                               cap, L.stream, stride, max_tiles);
         if (i < ktab.size() && v == 2 && has_seed[i])
             return launch_seed(dg, krep[i]->pat, seeds[i].desc, seeds[i].lds_bytes, (uint32_t)ktab[i].cbase,
-                               L.d_keys.p + koff[i], L.d_keys.p + soff[i], L.d_hkey.p, L.d_hmm.p, L.d_count.p,
-                               cand_slots(i), L.d_hkey.n, L.stream, stride, max_tiles);
+                               d_keys + koff[i], d_keys + soff[i], d_hkey, d_hmm, d_count,
+                               cand_slots(i), cap, L.stream, stride, max_tiles);
         if (i < ktab.size())
             return launch_fused(dg, krep[i]->pat, ktab[i].nw, ktab[i].gen, (uint32_t)ktab[i].cbase,
-                                L.d_keys.p + koff[i], ktab[i].units, 0, L.d_hkey.p, L.d_hmm.p, L.d_count.p,
-                                cand_slots(i), L.d_hkey.n, L.stream, v, stride, max_tiles);
+                                d_keys + koff[i], ktab[i].units, 0, d_hkey, d_hmm, d_count,
+                                cand_slots(i), cap, L.stream, v, stride, max_tiles);
         const size_t bg = i - ktab.size();
         const BulgeTable &t = btab[bg];
         if (v == 3 && has_bslice[bg])
-            return launch_bulge_slice(dg, t.pat, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg], bslice[bg].desc,
-                                      L.d_keys.p + bsoff[bg], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
-                                      L.d_hkey.n, L.stream, stride, max_tiles);
+            return launch_bulge_slice(dg, t.pat, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg], bslice[bg].desc,
+                                      d_keys + bsoff[bg], d_hkey, d_hmm, d_count, cand_slots(i),
+                                      cap, L.stream, stride, max_tiles);
         if (v == 2 && has_bidx[bg])
-            return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg], bidx[bg].desc,
-                                      L.d_keys.p + bioff[bg], L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i),
-                                      L.d_hkey.n, L.stream, stride, max_tiles);
-        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, L.d_keys.p + boff[bg],
-                            t.units, L.d_hkey.p, L.d_hmm.p, L.d_count.p, cand_slots(i), L.d_hkey.n, L.stream, v,
+            return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg], bidx[bg].desc,
+                                      d_keys + bioff[bg], d_hkey, d_hmm, d_count, cand_slots(i),
+                                      cap, L.stream, stride, max_tiles);
+        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg],
+                            t.units, d_hkey, d_hmm, d_count, cand_slots(i), cap, L.stream, v,
                             stride, max_tiles);
     };
     std::vector<int> variant(nlaunch, 0);
@@ -395,7 +421,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     } else if (tuner && tuner->force >= 0) {
         for (size_t i = 0; i < nlaunch; i++)
             variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : 1;
-    } else if (tuner && !(tuner->enabled && L.trials)) {
+    } else if (tuner && !(tuner->enabled && L.trials && !ext)) {
         for (size_t i = 0; i < nlaunch; i++)  // the nearest known decision, else the default
             if (!tuner->get_near(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
     } else if (tuner) {
@@ -429,9 +455,23 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             tried = true;
         }
         if (tried)  // trial hits and candidate counts are discarded
-            HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream),
-                    "hipMemsetAsync");
+            HIP_TRY(hipMemsetAsync(d_count, 0, count_n * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
         t_tune = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tt0).count();
+    }
+    if (ext) {
+        // eager launches and the hit counter back: no events, graphs or allocation
+        for (size_t i = 0; i < nlaunch; i++)
+            HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+        unsigned long long h = 0;
+        HIP_TRY(hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, L.stream), "hipMemcpyAsync(count)");
+        HIP_TRY(hipStreamSynchronize(L.stream), "fused");
+        nhits = h;
+        L.stats = cof_stats{};
+        L.stats.n_launches = nlaunch;
+        L.stats.n_hits = h;
+        L.stats.genome_len = dg.total;
+        L.launches.clear();
+        return COF_OK;
     }
     while (L.lev.size() < nlaunch + 1) {
         hipEvent_t e = nullptr;
@@ -439,7 +479,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         L.lev.push_back(e);
     }
     std::vector<std::string> kernels(nlaunch);
-    if (!replay && L.graphs && nlaunch) {
+    if (!replay && graphs && nlaunch) {
         // capture each launch as a graph of its own (kernels only: counters
         // are cleared before, outside the graphs; the events between them
         // time each launch); a failed capture falls back to eager launches
@@ -470,7 +510,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             L.graphs = false;
         }
     }
-    const bool use_graph = !L.graph.empty() && L.graphs && L.graph_key == gkey;
+    const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey;
     L.stats.graph = use_graph ? (replay ? 2 : 1) : 0;
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
     for (size_t i = 0; i < nlaunch; i++) {
@@ -487,7 +527,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     HIP_TRY(hipEventRecord(L.lev[nlaunch], L.stream), "hipEventRecord");
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
-    HIP_TRY(hipMemcpyAsync(raw.data(), L.d_count.p, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+    HIP_TRY(hipMemcpyAsync(raw.data(), d_count, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
                            L.stream),
             "hipMemcpyAsync(count)");
     HIP_TRY(hipStreamSynchronize(L.stream), "fused");
@@ -749,6 +789,26 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     L.stats.t_tune_ms = tune_ms;
     L.stats.genome_len = dg.total;
     *out = r;
+    return COF_OK;
+}
+
+int cof::run_search_noalloc(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
+                            Tuner *tuner, const ExtBufs &ext, uint64_t &nhits, std::string &err) {
+    nhits = 0;
+    if (qs.size() > 255) { err = "At most 255 queries per search."; return COF_E_INVALID; }
+    if (dg.total > kMaxGenomePositions) { err = "Genome too large for the hit key layout."; return COF_E_INVALID; }
+    if (ext.count_words < search_count_words(qs.size())) { err = "Counters exceed the scratch buffer."; return COF_E_NOMEM; }
+    std::vector<uint32_t> gbase(qs.size() + 1, 0);
+    uint64_t nkeys = 0;
+    for (size_t qi = 0; qi < qs.size(); qi++) {
+        gbase[qi] = (uint32_t)nkeys;
+        nkeys += qs[qi]->keys.size();
+        if (nkeys >= kMaxSearchKeys) { err = "Too many compare keys in one search."; return COF_E_INVALID; }
+    }
+    gbase[qs.size()] = (uint32_t)nkeys;
+    const int rc = search_batched(L, dg, qs, gbase, tuner, nhits, err, &ext);
+    if (rc) return rc;
+    if (nhits > ext.cap) { err = "Hit buffer too small."; return COF_E_NOMEM; }
     return COF_OK;
 }
 

@@ -288,6 +288,30 @@ struct Tuner {
     int save(const std::string &path, std::string &err);
 };
 
+// Device buffers a caller brings to a batched search: key tables, launch
+// counters (cleared by the caller), hit sort keys and mismatch counts.
+struct ExtBufs {
+    uint32_t *keys;
+    size_t keys_words;
+    unsigned long long *count;
+    size_t count_words;
+    uint64_t *hkey;
+    uint16_t *hmm;
+    uint64_t cap;
+};
+// Counter words a batched search of n queries needs (one launch per query at most).
+inline size_t search_count_words(size_t n) { return kCandBase + n * kCandWords; }
+// Batched search of `qs` into the caller's buffers: the tuner's decisions
+// (nearest known signature, no trials), eager launches, no device or
+// pinned allocation; *nhits = hits emitted (more than ext.cap: only the
+// first cap were written, COF_E_NOMEM).  Hits are unsorted.
+int run_search_noalloc(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs, Tuner *tuner,
+                       const ExtBufs &ext, uint64_t &nhits, std::string &err);
+
+// Sets up cof_search_noalloc's scratch array on `device` (the current
+// device): called once per context open.
+hipError_t noalloc_init(int device);
+
 // Runs `qs` over the genome on lane L; *out gets sorted hits.  Returns a
 // COF_* status with the message in err.  tuner may be null (default variants).
 int run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const std::vector<std::shared_ptr<Query>> &qs,

@@ -166,6 +166,25 @@ int cof_query_info(const cof_query *q, const char **chrdir, uint32_t *pattern_le
  *      expanded on the device). */
 int cof_search(cof_ctx *ctx, cof_query *const *queries, int n_queries, int mode, cof_result **out);
 int cof_stats_get(const cof_ctx *ctx, cof_stats *out);
+/* Batched search that allocates nothing.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  As
+ * cof_search in COF_MODE_BATCHED, but no device or pinned host memory is
+ * allocated (no hipMalloc / hipHostMalloc / hipFree, no library
+ * temporaries, no graphs, events or autotuner trials: launch variants come
+ * from the context's tuner decisions); only host memory for the queries'
+ * tables.  The key tables and launch counters live in the library's
+ * scratch array of 8 MiB per GPU (COF_E_NOMEM when a search needs more);
+ * every call on one GPU shares it, so calls that overlap in time on one GPU
+ * race.  hit_keys / hit_mm are caller-owned DEVICE buffers of cap entries;
+ * the hits are written unsorted, hit_keys[i] = global key index << 40 |
+ * window start << 1 | strand (0 '+', 1 '-'), where a query's global key
+ * indices start at the sum of the key counts of the queries before it
+ * (cof_query_info), and hit_mm[i] = its mismatches.  *n_hits = hits found;
+ * more than cap: only cap are written and COF_E_NOMEM is returned.  The
+ * call returns after the hits are written. */
+int cof_search_noalloc(cof_ctx *ctx, cof_query *const *queries, int n_queries, uint64_t *hit_keys,
+                       uint16_t *hit_mm, uint64_t cap, uint64_t *n_hits);
 /* Autotuner decisions of the context so far, one "signature<TAB>variant" line
  * each (no reference counterpart: the reference's launch shape is fixed).
  * Caller frees *out with cof_free. */
