@@ -12,8 +12,9 @@
  *
  * Parity status per function (what pins it besides this literal reading):
  *   orc_read_fasta / orc_read_twobit  pinned against the reference's own
- *       read_fasta.cpp / read_twobit.cpp compiled into oracle/_ref
- *       (tests/test_oracle_readers.py) and against the UCSC 2bit spec.
+ *       read_fasta.cpp / read_twobit.cpp compiled into oracle/_ref (when
+ *       the reference tree is present; tests/test_readers.py), against
+ *       handcrafted files and against the UCSC 2bit layout.
  *   iupac_mismatch      pinned by the closed-form IUPAC table test
  *       (README.md:139-151 base sets; tests/test_oracle_core.py).
  *   orc_finder / orc_comparer  pinned by brute force over the IUPAC sets,
@@ -21,13 +22,20 @@
  *       sites and contig-separator invariants (tests/test_oracle_core.py).
  *   orc_parse_input     pinned by key-count closed forms and by the
  *       gap-removal relation of every DNA/RNA key to its guide
- *       (tests/test_oracle_parse.py).
- *   format_hit          pinned for forward-PAM '+' records and reversed-PAM
- *       records by re-aligning crRNA/DNA strings to the genome text at the
- *       printed location (tests/test_oracle_format.py).  The forward-PAM '-'
- *       strand branch with bulge slack (cas-offinder.cpp:407-409, 417-419,
- *       427-430, 437-442) prints strings that no alignment property fixes:
- *       parity unpinned there beyond the literal reading.
+ *       (tests/test_parse.py).
+ *   format_hit          forward-PAM '+' records and reversed-PAM '+' X / RNA
+ *       records: pinned by re-aligning the crRNA/DNA strings to the genome
+ *       text at the printed location.  Reversed-orientation records
+ *       (reversed-PAM records of both strands, forward-PAM '-' records,
+ *       cas-offinder.cpp:403-449): pinned by the window at the printed
+ *       location, the trim of each type, lowercase on the IUPAC mismatches
+ *       of aligned columns (with the trimmed tail's mismatches making up
+ *       Mismatches) and the crRNA as the parsed key minus the info's
+ *       columns (tests/test_format.py).  Unpinned beyond the literal
+ *       reading: which columns the reference chose to keep there (the D or
+ *       D + size trim taken from the end, the gap at the key's bulge index
+ *       rather than at its inserted columns), so the slack N content of
+ *       those crRNA strings is fixed by no property.
  */
 #define _GNU_SOURCE
 #include "casoff_oracle.h"
