@@ -269,12 +269,15 @@ def serve_bench(ctx, n_requests, qps, seed=3, verify=True):
         tickets.append(srv.submit(qs))
     done = [srv.poll(t, wait=True) for t in tickets]
     wall = time.perf_counter() - t0
+    st = srv.stats()
     srv.stop()
     lat = sorted(ms for _, ms in done)
     pct = lambda q: lat[min(len(lat) - 1, int(round(q * (len(lat) - 1))))]
     out = {"requests": n_requests, "qps_offered": qps, "guides": n_guides,
            "p50_ms": round(pct(0.50), 2), "p99_ms": round(pct(0.99), 2), "max_ms": round(lat[-1], 2),
-           "wall_s": round(wall, 3), "guide_x_gbp_per_s": round(n_guides * ctx.info()[0] / 1e9 / wall, 2)}
+           "wall_s": round(wall, 3), "guide_x_gbp_per_s": round(n_guides * ctx.info()[0] / 1e9 / wall, 2),
+           "batches": st["batches"], "lane_hits": st["lane_hits"], "lane_grows": st["lane_grows"],
+           "admission_waits": st["admission_waits"]}
     if verify:
         same_alone, same_plain, checked_plain = 0, 0, 0
         for (res, _), qs, (_, _, n) in zip(done, queries, reqs):
@@ -287,6 +290,39 @@ def serve_bench(ctx, n_requests, qps, seed=3, verify=True):
                 same_plain += int(len(got) == len(plain) and bool((got == plain).all()))
         out["bit_exact_vs_alone"] = "%d/%d" % (same_alone, n_requests)
         out["bit_exact_vs_plain"] = "%d/%d (requests of <= 8 guides)" % (same_plain, checked_plain)
+    return out
+
+
+def serve_capacity(ctx, n_requests, levels, p99_ms, plain_requests):
+    """Served capacity: the serving stream at each offered rate in `levels`
+    (at least n_requests requests, about 2 s of arrivals); capacity is the highest achieved guide×Gbp/s whose p99 latency is within
+    p99_ms.  Beside it, the plain path serving the first plain_requests
+    requests one after another (no queueing: its rate is an upper bound of
+    what plain serving sustains)."""
+    sweep = []
+    for qps in levels:  # about 2 s of arrivals at each rate, so that the rate is sustained
+        r = serve_bench(ctx, max(n_requests, int(2 * qps)), qps, verify=False)
+        log("serving %.0f QPS: p50 %.1f ms, p99 %.1f ms, %.0f guide×Gbp/s" %
+            (qps, r["p50_ms"], r["p99_ms"], r["guide_x_gbp_per_s"]))
+        sweep.append(r)
+    ok = [r for r in sweep if r["p99_ms"] <= p99_ms]
+    out = {"p99_target_ms": p99_ms, "sweep": sweep,
+           "capacity_guide_x_gbp_per_s": max((r["guide_x_gbp_per_s"] for r in ok), default=None),
+           "capacity_qps": max((r["qps_offered"] for r in ok), default=None)}
+    if plain_requests > 0:
+        reqs = workloads.serving_stream(plain_requests, levels[0], "synthetic", seed=3)
+        guides, busy = 0, 0.0
+        for _, groups, n in reqs:
+            qs = [ctx.query(t) for _, t in groups]
+            t0 = time.perf_counter()
+            ctx.search(qs, casoff.MODE_PLAIN).records()
+            busy += time.perf_counter() - t0
+            guides += n
+        out["plain"] = {"requests": plain_requests, "guides": guides, "busy_s": round(busy, 3),
+                        "guide_x_gbp_per_s": round(guides * ctx.info()[0] / 1e9 / busy, 2)}
+        if out["capacity_guide_x_gbp_per_s"]:
+            out["capacity_over_plain"] = round(out["capacity_guide_x_gbp_per_s"] / out["plain"]["guide_x_gbp_per_s"], 1)
+        log("plain serving: %.0f guide×Gbp/s over %d requests" % (out["plain"]["guide_x_gbp_per_s"], plain_requests))
     return out
 
 
@@ -304,6 +340,10 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=20.0)
+    ap.add_argument("--serve-sweep", default="20,40,80,160,320",
+                    help="offered request rates of the served-capacity sweep ('' to skip)")
+    ap.add_argument("--serve-p99-ms", type=float, default=100.0, help="p99 latency bound of the served capacity")
+    ap.add_argument("--serve-plain", type=int, default=12, help="requests the plain path serves beside the sweep")
     ap.add_argument("--calibrate-copy", type=int, default=0, metavar="BYTES",
                     help="only run a device copy of BYTES (PMC byte-counter calibration)")
     args = ap.parse_args()
@@ -360,6 +400,10 @@ def main():
         log("serving: %d requests at %.1f QPS" % (args.serve_requests, args.serve_qps))
         out["serving"] = serve_bench(ctx, args.serve_requests, args.serve_qps)
         log("serving: p50 %.1f ms, p99 %.1f ms" % (out["serving"]["p50_ms"], out["serving"]["p99_ms"]))
+        levels = [float(x) for x in args.serve_sweep.split(",") if x]
+        if levels:
+            out["serving_capacity"] = serve_capacity(ctx, args.serve_requests, levels, args.serve_p99_ms,
+                                                     args.serve_plain)
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(ctx, texts)
     print(json.dumps(out), flush=True)

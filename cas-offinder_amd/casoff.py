@@ -66,6 +66,16 @@ class ServerCfg(C.Structure):
     _fields_ = [("mem_budget_bytes", C.c_uint64), ("max_batch_queries", C.c_uint32)]
 
 
+class ServerStats(C.Structure):
+    _fields_ = [("lane_hits", C.c_uint64), ("batches", C.c_uint64), ("requests", C.c_uint64),
+                ("queries", C.c_uint64), ("hits", C.c_uint64), ("lane_grows", C.c_uint64),
+                ("admission_waits", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
+                ("hits_per_key", C.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class CofError(RuntimeError):
     pass
 
@@ -116,6 +126,7 @@ def lib():
             "cof_server_submit": (i32, [vp, C.POINTER(vp), i32, C.POINTER(u64)]),
             "cof_server_poll": (i32, [vp, u64, i32, C.POINTER(vp), C.POINTER(C.c_double)]),
             "cof_server_stop": (i32, [vp]),
+            "cof_server_stats_get": (i32, [vp, C.POINTER(ServerStats)]),
         }
         for name, (res, args) in sig.items():
             f = getattr(L, name)
@@ -370,6 +381,12 @@ class Server:
         if rc:
             raise CofError(self.ctx.error())
         return Result(self.ctx, out), lat.value
+
+    def stats(self):
+        st = ServerStats()
+        if lib().cof_server_stats_get(self._p, C.byref(st)):
+            raise CofError("server stats failed")
+        return st.as_dict()
 
     def stop(self):
         if self._p.value:

@@ -1,13 +1,18 @@
 // Serving loop: continuous batching of queued queries over the resident genome.
 //
 // Requests (one or more queries each) wait in a FIFO.  Each worker owns a
-// lane (its own HIP stream and buffers); an idle worker admits requests from
-// the head of the queue while the merged batch stays within max_batch_queries
-// and the in-flight estimate of key-table and hit-buffer bytes stays within
-// the memory budget, runs one batched search for all admitted queries, and
-// splits the records back per request.  Requests that arrive while a batch is
-// on the GPU are admitted by the next free worker, so batches on different
-// lanes overlap on the device.
+// lane (its own HIP stream and buffers, hit buffers sized at start from the
+// memory budget); an idle worker admits requests from the head of the queue
+// while the merged batch stays within max_batch_queries, its estimated hits
+// fit the lane's hit buffers, and the in-flight estimate of key-table and
+// hit bytes of all running batches stays within the budget; it runs one
+// batched search for all admitted queries and splits the records back per
+// request.  Requests that arrive while a batch is on the GPU are admitted by
+// the next free worker, so batches on different lanes overlap on the device.
+// The hit estimate of a request is its compare-key count times the running
+// average of hits per compare key of the batches served so far; a batch that
+// finds more hits than its lane holds grows the lane's buffers and reruns
+// (counted in cof_server_stats).
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -27,18 +32,25 @@ struct Request {
     uint64_t ticket = 0;
     std::vector<std::shared_ptr<Query>> qs;
     clk::time_point t_submit;
-    uint64_t est_bytes = 0;
+    uint64_t keys = 0;         // compare keys of the request's queries
+    uint64_t table_bytes = 0;  // estimated key / bulge table bytes
     cof_result *result = nullptr;
     double latency_ms = 0;
     int status = 1;  // 1 pending, 0 done, < 0 error
     std::string err;
 };
 
-// Admission estimate: the query's key table plus a hit allowance of 4096
-// records per compare key, at 20 device bytes per hit (8-byte sort key and
-// 2-byte mismatch count, emitted and sorted copies).
-uint64_t estimate_bytes(const Query &q) {
-    return q.key_words.size() * 4ull + q.keys.size() * 4096ull * 20ull;
+// Device bytes per hit in a lane: emitted and sorted sort keys (8 + 8),
+// mismatch counts (2 + 2) and the packed copy (4 + 1).
+constexpr uint64_t kLaneBytesPerHit = 25;
+constexpr uint64_t kMinLaneHits = 1ull << 20, kMaxLaneHits = 1ull << 26;
+
+// Table bytes of a query: its key table, or for bulge queries a record and a
+// variant block per guide (entries of its splits plus the family headers).
+uint64_t table_bytes(const Query &q) {
+    uint64_t b = q.key_words.size() * 4ull;
+    for (const BulgeGuide &g : q.bulge) b += (g.ent.size() + 128) * 4ull;
+    return b;
 }
 
 }  // namespace
@@ -52,10 +64,14 @@ struct cof_server {
     std::map<uint64_t, Request *> requests;
     uint64_t next_ticket = 1;
     uint64_t inflight_bytes = 0;
+    uint64_t lane_hits = 0;          // hit capacity of each lane
+    double hits_per_key = 1024.0;    // running average over served batches
+    cof_server_stats st{};
     bool stopping = false;
     std::vector<std::unique_ptr<Lane>> lanes;
     std::vector<std::thread> workers;
 
+    uint64_t est_hits(const Request &r) const { return (uint64_t)(r.keys * hits_per_key) + 1; }
     void worker(Lane *L);
 };
 
@@ -63,7 +79,7 @@ void cof_server::worker(Lane *L) {
     (void)hipSetDevice(ctx->device);
     for (;;) {
         std::vector<Request *> batch;
-        uint64_t bytes = 0;
+        uint64_t bytes = 0, hits = 0, keys = 0;
         size_t nq = 0;
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -74,12 +90,19 @@ void cof_server::worker(Lane *L) {
                         const bool first = batch.empty();
                         if (!first && nq + r->qs.size() > cfg.max_batch_queries) break;
                         if (nq + r->qs.size() > 255) break;
-                        // a request that alone exceeds the budget runs when nothing else is in flight
-                        const bool fits = inflight_bytes + bytes + r->est_bytes <= cfg.mem_budget_bytes;
-                        if (!fits && !(first && inflight_bytes == 0)) break;
+                        const uint64_t h = est_hits(*r), rb = r->table_bytes + h * kLaneBytesPerHit;
+                        // the first request of a batch is admitted whatever its size when
+                        // nothing else is in flight (a larger one grows the lane)
+                        const bool fits = hits + h <= lane_hits && inflight_bytes + bytes + rb <= cfg.mem_budget_bytes;
+                        if (!fits && !(first && inflight_bytes == 0)) {
+                            if (first) st.admission_waits++;  // the head waits for running batches
+                            break;
+                        }
                         pending.pop_front();
                         batch.push_back(r);
-                        bytes += r->est_bytes;
+                        bytes += rb;
+                        hits += h;
+                        keys += r->keys;
                         nq += r->qs.size();
                     }
                     if (!batch.empty()) break;
@@ -88,12 +111,16 @@ void cof_server::worker(Lane *L) {
                 cv_work.wait(lk);
             }
             inflight_bytes += bytes;
+            st.peak_inflight_bytes = std::max(st.peak_inflight_bytes, inflight_bytes);
         }
         std::vector<std::shared_ptr<Query>> qs;
         for (Request *r : batch) qs.insert(qs.end(), r->qs.begin(), r->qs.end());
         cof_result *res = nullptr;
         std::string err;
+        const uint64_t cap_before = L->d_hkey.n;
         const int rc = run_search(*L, ctx->dg, ctx->slab_positions, qs, COF_MODE_BATCHED, &ctx->tuner, &res, err);
+        const bool grew = L->d_hkey.n > cap_before;
+        const uint64_t found = rc == COF_OK ? res->hits.size() : 0;
         // split the merged result per request, query indices rebased to 0
         std::vector<cof_result *> parts(batch.size(), nullptr);
         if (rc == COF_OK) {
@@ -130,6 +157,14 @@ void cof_server::worker(Lane *L) {
                 r->latency_ms = std::chrono::duration<double, std::milli>(now - r->t_submit).count();
             }
             inflight_bytes -= bytes;
+            st.batches++;
+            st.requests += batch.size();
+            st.queries += nq;
+            st.hits += found;
+            st.lane_grows += grew ? 1 : 0;
+            if (rc == COF_OK && keys)  // running average of hits per key, weighted by keys
+                hits_per_key = std::max(1.0, 0.7 * hits_per_key + 0.3 * ((double)found / (double)keys));
+            st.hits_per_key = hits_per_key;
         }
         cv_done.notify_all();
         cv_work.notify_all();
@@ -149,17 +184,18 @@ int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out) 
     s->cfg.max_batch_queries = cfg && cfg->max_batch_queries ? std::min<uint32_t>(cfg->max_batch_queries, 255) : 64;
     (void)hipSetDevice(ctx->device);
     const int nlanes = 2;
+    // each lane's hit buffers: its share of the budget, in [1 M, 64 M] hits
+    s->lane_hits = std::min(kMaxLaneHits, std::max(kMinLaneHits, s->cfg.mem_budget_bytes / nlanes / kLaneBytesPerHit));
+    s->st.lane_hits = s->lane_hits;
     for (int i = 0; i < nlanes; i++) {
         auto L = std::make_unique<Lane>();
         L->graphs = false;  // merged batches differ from call to call: nothing to replay
-        L->trials = false;  // tuned variants of the context are used, untuned ones default
-        // hit buffers sized up front: a first large request then does not pay
-        // for an overflow rerun
-        constexpr uint64_t kLaneHits = 16ull << 20;
-        if (L->init() != hipSuccess || L->d_hkey.reserve(kLaneHits) != hipSuccess ||
-            L->d_hmm.reserve(kLaneHits) != hipSuccess || L->d_hkey2.reserve(kLaneHits) != hipSuccess ||
-            L->d_hmm2.reserve(kLaneHits) != hipSuccess || L->h_hkey.reserve(kLaneHits) != hipSuccess ||
-            L->h_hmm.reserve(kLaneHits) != hipSuccess) {
+        L->trials = false;  // the tuner file's (nearest) decisions, no trials
+        const uint64_t n = s->lane_hits;
+        if (L->init() != hipSuccess || L->d_hkey.reserve(n) != hipSuccess || L->d_hmm.reserve(n) != hipSuccess ||
+            L->d_hkey2.reserve(n) != hipSuccess || L->d_hmm2.reserve(n) != hipSuccess ||
+            L->d_pos32.reserve(n) != hipSuccess || L->d_ms.reserve(n) != hipSuccess ||
+            L->h_pos32.reserve(n) != hipSuccess || L->h_ms.reserve(n) != hipSuccess) {
             for (auto &x : s->lanes) x->release();
             L->release();
             delete s;
@@ -179,7 +215,8 @@ int cof_server_submit(cof_server *s, cof_query *const *queries, int n_queries, u
     for (int i = 0; i < n_queries; i++) {
         if (!queries[i]) { delete r; return COF_E_INVALID; }
         r->qs.push_back(queries[i]->q);
-        r->est_bytes += estimate_bytes(*queries[i]->q);
+        r->keys += queries[i]->q->keys.size();
+        r->table_bytes += table_bytes(*queries[i]->q);
     }
     r->t_submit = clk::now();
     {
@@ -210,6 +247,13 @@ int cof_server_poll(cof_server *s, uint64_t ticket, int wait, cof_result **out, 
     s->requests.erase(it);
     delete r;
     return st;
+}
+
+int cof_server_stats_get(cof_server *s, cof_server_stats *out) {
+    if (!s || !out) return COF_E_INVALID;
+    std::lock_guard<std::mutex> lk(s->mu);
+    *out = s->st;
+    return COF_OK;
 }
 
 int cof_server_stop(cof_server *s) {

@@ -229,9 +229,11 @@ int cof_format_hits(cof_ctx *ctx, const cof_query *q, const cof_hit *hits, size_
 void cof_free(void *p);
 
 /* ---- serving (no reference counterpart: the reference runs one input per
- *      process).  A server owns the context on its own thread, admits queued
- *      queries while the device-memory budget for key tables and hit buffers
- *      allows, and merges admitted queries into batched passes as they come. */
+ *      process).  A server owns the context on its own threads (two lanes,
+ *      each with hit buffers of its share of mem_budget_bytes), admits queued
+ *      requests while their estimated key-table and hit bytes fit a lane and
+ *      the budget, and merges admitted queries into batched passes as they
+ *      come. */
 typedef struct {
     uint64_t mem_budget_bytes;   /* key tables + hit buffers of in-flight queries */
     uint32_t max_batch_queries;  /* queries merged into one pass */
@@ -246,6 +248,21 @@ int cof_server_submit(cof_server *s, cof_query *const *queries, int n_queries, u
  * 1: pending, < 0: error.  A done or failed ticket is forgotten. */
 int cof_server_poll(cof_server *s, uint64_t ticket, int wait, cof_result **out, double *latency_ms);
 int cof_server_stop(cof_server *s);
+/* What the server did so far: each lane's hit capacity (its share of
+ * mem_budget_bytes at 25 device bytes per hit, 1 M to 64 M hits), served
+ * batches / requests / queries / hits, batches whose hits outgrew the lane
+ * (grown and rerun), admissions that waited for running batches, the peak
+ * estimate of in-flight bytes, and the running average of hits per compare
+ * key that sizes the admission estimate. */
+typedef struct {
+    uint64_t lane_hits;
+    uint64_t batches, requests, queries, hits;
+    uint64_t lane_grows;
+    uint64_t admission_waits;
+    uint64_t peak_inflight_bytes;
+    double hits_per_key;
+} cof_server_stats;
+int cof_server_stats_get(cof_server *s, cof_server_stats *out);
 
 #ifdef __cplusplus
 }

@@ -24,14 +24,23 @@ def test_served_results_match_alone(tmp_path):
         alone.append([r.format(i) for i in range(len(qs))])
     srv = casoff.Server(ctx, mem_budget_bytes=4 << 20, max_batch_queries=8)
     tickets = [srv.submit(qs) for qs in queries]
-    lat = []
+    lat, n_records = [], 0
     for t, exp in zip(tickets, alone):
         res, ms = srv.poll(t, wait=True)
         lat.append(ms)
+        n_records += len(res.records())
         got = [res.format(i) for i in range(len(exp))]
         assert [H.record_lines(x) for x in got] == [H.record_lines(x) for x in exp]
+    st = srv.stats()
     srv.stop()
     assert all(x > 0 for x in lat)
+    # every request and query was served once, and the merged batches' hits
+    # are the records of the split results
+    assert st["requests"] == len(queries)
+    assert st["queries"] == sum(len(qs) for qs in queries)
+    assert st["hits"] == n_records > 0
+    assert st["lane_hits"] == 1 << 20  # the smallest lane: 4 MiB is below its share
+    assert 1 <= st["batches"] <= len(queries) and st["hits_per_key"] >= 1.0
     ctx.close()
 
 
