@@ -333,8 +333,15 @@ int64_t cof_result_count(const cof_result *r) { return r ? (int64_t)r->hits.size
 
 int cof_result_records(const cof_result *r, cof_hit *out, size_t cap) {
     if (!r || (!out && cap)) return COF_E_INVALID;
-    const size_t n = std::min(cap, r->hits.size());
-    if (n) memcpy(out, r->hits.data(), n * sizeof(cof_hit));
+    // query by query (each query's run is sorted; the runs lie in launch order)
+    size_t at = 0;
+    for (size_t q = 0; q < r->queries.size() && at < cap; q++) {
+        const size_t b = r->qbegin[q], n = std::min(cap - at, r->qend[q] - b);
+        parallel_for(n, 1 << 20, [&](uint64_t x, uint64_t y) {
+            memcpy(out + at + x, r->hits.data() + b + x, (y - x) * sizeof(cof_hit));
+        });
+        at += n;
+    }
     return COF_OK;
 }
 
@@ -343,7 +350,7 @@ static int format_query(cof_ctx *c, const cof_result *r, int qi, std::string &te
     // hits index the genome they were searched on
     if (r->genome_serial != c->genome_serial || c->genome.files.empty())
         return c->fail(COF_E_INVALID, "The result was searched on another genome than the context's.");
-    const size_t b = r->qbegin[qi], e = r->qbegin[qi + 1];
+    const size_t b = r->qbegin[qi], e = r->qend[qi];
     static_assert(sizeof(cof_hit) == sizeof(DevHit), "hit layout");
     format_hits(c->genome, *r->queries[qi], reinterpret_cast<const DevHit *>(r->hits.data()) + b, e - b, text, sum);
     return COF_OK;

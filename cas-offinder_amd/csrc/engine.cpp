@@ -4,6 +4,10 @@
 #include <algorithm>
 #include <chrono>
 #include <cstring>
+#include <functional>
+#include <thread>
+#include <mutex>
+#include <condition_variable>
 #include <map>
 
 using namespace cof;
@@ -281,9 +285,164 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
 //  - the other queries with the same padded pattern form a key group whose
 //    compare keys go into one table (fused_kernel).
 // Records carry global key indices.
+// Per-launch collection (pipe): after each launch a one-thread kernel copies
+// the hit counter into pinned memory, so the host learns where the launch's
+// hits end.  The host keeps one launch ahead of the device: once launch i is
+// done (launch i + 1 already queued), it queues the radix sort of launch i's
+// range behind launch i + 1 on the lane's stream (the sort gets the whole
+// GPU, between launches), then the range's copy on the second stream, and
+// host threads decode copied ranges into the result at the same offsets
+// while later launches run.  Every query's keys go to exactly one launch and
+// every key of a launch belongs to its queries, so each query's hits are one
+// sorted run inside its launch's range.  enqueue(i) queues launch i and its
+// counter snapshot (event L.sev[i]).
+static int collect_ranges(Lane &L, const std::vector<std::shared_ptr<Query>> &qs, const std::vector<uint32_t> &gbase,
+                          size_t nlaunch, uint64_t cap, unsigned end_bit,
+                          const std::function<hipError_t(size_t)> &enqueue, cof_result *&res, std::string &err) {
+    res = nullptr;
+    const size_t nq = qs.size();
+    auto r = std::make_unique<cof_result>();
+    r->hits.resize(cap);  // untouched beyond the hits found (no zero fill)
+    r->qbegin.assign(nq, 0);
+    r->qend.assign(nq, 0);
+    std::vector<uint64_t> lo(nlaunch, 0), hi(nlaunch, 0);
+    double t_decode = 0;
+    auto decode = [&](size_t i) {
+        const uint64_t b = lo[i], e = hi[i];
+        if (e <= b) return;
+        const auto td = std::chrono::steady_clock::now();
+        const uint64_t *hk = L.h_hkey.p;
+        const uint16_t *hm = L.h_hmm.p;
+        parallel_for(e - b, 1 << 16, [&](uint64_t x0, uint64_t x1) {
+            x0 += b, x1 += b;
+            size_t q = (size_t)(std::upper_bound(gbase.begin(), gbase.end() - 1, (uint32_t)(hk[x0] >> kHitKeyShift)) -
+                                gbase.begin()) - 1;
+            for (uint64_t x = x0; x < x1; x++) {
+                const uint64_t k = hk[x];
+                const uint32_t gk = (uint32_t)(k >> kHitKeyShift);
+                while (q + 1 < nq && gk >= gbase[q + 1]) q++;  // keys are sorted
+                cof_hit &h = r->hits[x];
+                h.pos = (k >> 1) & (kMaxGenomePositions - 1);
+                h.strand = (uint8_t)(k & 1);
+                h.key = gk - gbase[q];
+                h.mm = hm[x];
+                h.query = (uint8_t)q;
+            }
+        });
+        for (size_t q = 0; q < nq; q++) {
+            const uint64_t *a = std::lower_bound(hk + b, hk + e, (uint64_t)gbase[q] << kHitKeyShift);
+            const uint64_t *z = std::lower_bound(a, hk + e, (uint64_t)gbase[q + 1] << kHitKeyShift);
+            if (z > a) r->qbegin[q] = (size_t)(a - hk), r->qend[q] = (size_t)(z - hk);
+        }
+        t_decode += std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - td).count();
+    };
+    // decoder thread: ranges in launch order, each once its copy is done, so
+    // that the loop below keeps queueing launches meanwhile
+    std::mutex mu;
+    std::condition_variable cv;
+    size_t ready = 0;  // ranges whose copies are queued (lo / hi set)
+    bool stop = false;
+    hipError_t dec_err = hipSuccess;
+    std::thread decoder([&] {
+        for (size_t d = 0; d < nlaunch; d++) {
+            {
+                std::unique_lock<std::mutex> lk(mu);
+                cv.wait(lk, [&] { return ready > d || stop; });
+                if (ready <= d) return;
+            }
+            if (hi[d] > lo[d]) {
+                const hipError_t x = hipEventSynchronize(L.cev2[3 * d + 2]);
+                if (x != hipSuccess) {
+                    dec_err = x;
+                    return;
+                }
+            }
+            decode(d);
+        }
+    });
+    auto finish = [&](bool abort) {
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            stop = abort;
+            if (!abort) ready = nlaunch;
+        }
+        cv.notify_all();
+        decoder.join();
+    };
+    struct Joiner {  // error returns below: stop the decoder first
+        std::function<void()> f;
+        ~Joiner() { if (f) f(); }
+    } joiner{[&] { finish(true); }};
+    size_t tmp_bytes = L.d_sort_tmp.n;
+    uint64_t prev = 0;
+    bool over = false;
+    auto t_last = std::chrono::steady_clock::now();
+    if (nlaunch) HIP_TRY(enqueue(0), "batched launch");
+    for (size_t i = 0; i < nlaunch; i++) {
+        if (i + 1 < nlaunch) HIP_TRY(enqueue(i + 1), "batched launch");
+        HIP_TRY(hipEventSynchronize(L.sev[i]), "batched launch");
+        if (over) continue;  // the remaining launches still count their hits for the rerun
+        const uint64_t e = reinterpret_cast<volatile unsigned long long *>(L.h_snap.p)[i];
+        lo[i] = prev;
+        hi[i] = std::max(e, prev);
+        if (e > cap) {
+            over = true;
+            continue;
+        }
+        if (i + 1 == nlaunch) t_last = std::chrono::steady_clock::now();
+        if (e > prev) {
+            const uint64_t n = e - prev;
+            HIP_TRY(hipEventRecord(L.cev2[3 * i], L.stream), "hipEventRecord");
+            const uint64_t *sk = L.d_hkey.p;
+            const uint16_t *sm = L.d_hmm.p;
+            if (n > 1) {
+                HIP_TRY(sort_hits(L.d_sort_tmp.p, &tmp_bytes, L.d_hkey.p + prev, L.d_hkey2.p + prev, L.d_hmm.p + prev,
+                                  L.d_hmm2.p + prev, n, end_bit, L.stream),
+                        "radix sort");
+                sk = L.d_hkey2.p;
+                sm = L.d_hmm2.p;
+            }
+            HIP_TRY(hipEventRecord(L.cev2[3 * i + 1], L.stream), "hipEventRecord");
+            HIP_TRY(hipStreamWaitEvent(L.stream2, L.cev2[3 * i + 1], 0), "hipStreamWaitEvent");
+            HIP_TRY(hipMemcpyAsync(L.h_hkey.p + prev, sk + prev, n * 8, hipMemcpyDeviceToHost, L.stream2),
+                    "hipMemcpy(hits)");
+            HIP_TRY(hipMemcpyAsync(L.h_hmm.p + prev, sm + prev, n * 2, hipMemcpyDeviceToHost, L.stream2),
+                    "hipMemcpy(hits)");
+            HIP_TRY(hipEventRecord(L.cev2[3 * i + 2], L.stream2), "hipEventRecord");
+        }
+        prev = hi[i];
+        {
+            std::lock_guard<std::mutex> lk(mu);
+            ready = i + 1;
+        }
+        cv.notify_all();
+    }
+    joiner.f = nullptr;
+    finish(over);
+    if (over) {
+        HIP_TRY(hipStreamSynchronize(L.stream2), "hit copy");
+        return COF_OK;  // the caller grows the buffers and searches again
+    }
+    HIP_TRY(dec_err, "hit copy");
+    double t_sort = 0, t_copy = 0;
+    for (size_t i = 0; i < nlaunch; i++)
+        if (hi[i] > lo[i]) {
+            t_sort += ms_between(L.cev2[3 * i], L.cev2[3 * i + 1]);
+            t_copy += ms_between(L.cev2[3 * i + 1], L.cev2[3 * i + 2]);
+        }
+    r->hits.resize(prev);
+    r->queries = qs;
+    L.stats.t_sort_ms = t_sort;
+    L.stats.t_copy_ms = t_copy;
+    L.stats.t_decode_ms = t_decode;
+    L.stats.t_collect_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_last).count();
+    res = r.release();
+    return COF_OK;
+}
+
 int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_ptr<Query>> &qs,
                    const std::vector<uint32_t> &gbase, Tuner *tuner, uint64_t &nhits, std::string &err,
-                   const ExtBufs *ext = nullptr) {
+                   const ExtBufs *ext = nullptr, cof_result **pres = nullptr) {
     // the plan of the lane's previous search is reused for the same query
     // objects (it holds them, so their addresses identify them)
     if (!(L.plan && L.plan->qs == qs && L.plan->gbase == gbase)) {
@@ -473,10 +632,35 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         L.launches.clear();
         return COF_OK;
     }
-    while (L.lev.size() < nlaunch + 1) {
-        hipEvent_t e = nullptr;
-        HIP_TRY(hipEventCreate(&e), "hipEventCreate");
-        L.lev.push_back(e);
+    const bool pipe = pres != nullptr;
+    if (pres) *pres = nullptr;
+    auto grow_events = [&](std::vector<hipEvent_t> &v, size_t n) -> hipError_t {
+        while (v.size() < n) {
+            hipEvent_t e = nullptr;
+            const hipError_t x = hipEventCreate(&e);
+            if (x != hipSuccess) return x;
+            v.push_back(e);
+        }
+        return hipSuccess;
+    };
+    HIP_TRY(grow_events(L.lev, 2 * nlaunch), "hipEventCreate");
+    unsigned end_bit = kHitKeyShift + 1;
+    if (pipe) {
+        // everything the collection needs is allocated before the launches
+        HIP_TRY(grow_events(L.sev, nlaunch), "hipEventCreate");
+        HIP_TRY(grow_events(L.cev2, 3 * nlaunch), "hipEventCreate");
+        HIP_TRY(L.h_snap.reserve(nlaunch), "hipHostMalloc(counters)");
+        HIP_TRY(L.d_hkey2.reserve(cap), "hipMalloc(sort)");
+        HIP_TRY(L.d_hmm2.reserve(cap), "hipMalloc(sort)");
+        HIP_TRY(L.h_hkey.reserve(cap), "hipHostMalloc(hits)");
+        HIP_TRY(L.h_hmm.reserve(cap), "hipHostMalloc(hits)");
+        unsigned keybits = 1;
+        while ((1ull << keybits) < (uint64_t)gbase.back()) keybits++;
+        end_bit = kHitKeyShift + keybits;
+        size_t tmp_bytes = 0;
+        HIP_TRY(sort_hits(nullptr, &tmp_bytes, L.d_hkey.p, L.d_hkey2.p, L.d_hmm.p, L.d_hmm2.p, cap, end_bit, L.stream2),
+                "radix sort size");
+        HIP_TRY(L.d_sort_tmp.reserve(tmp_bytes), "hipMalloc(sort)");
     }
     std::vector<std::string> kernels(nlaunch);
     if (!replay && graphs && nlaunch) {
@@ -513,18 +697,39 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey;
     L.stats.graph = use_graph ? (replay ? 2 : 1) : 0;
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
-    for (size_t i = 0; i < nlaunch; i++) {
-        HIP_TRY(hipEventRecord(L.lev[i], L.stream), "hipEventRecord");
+    // launch i between its timing events (and, collecting per launch, the
+    // hit counter snapshot after it)
+    auto enqueue = [&](size_t i) -> hipError_t {
+        hipError_t e = hipEventRecord(L.lev[2 * i], L.stream);
+        if (e != hipSuccess) return e;
         if (use_graph) {
-            HIP_TRY(hipGraphLaunch(L.graph[i], L.stream), "hipGraphLaunch");
+            e = hipGraphLaunch(L.graph[i], L.stream);
             kernels[i] = L.graph_kernels[i];
         } else {
             reset_launched_kernel();
-            HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+            e = launch(i, variant[i], 1, 0);
             kernels[i] = last_launched_kernel();
         }
+        if (e != hipSuccess) return e;
+        if ((e = hipEventRecord(L.lev[2 * i + 1], L.stream)) != hipSuccess) return e;
+        if (pipe) {
+            if ((e = snap_count(d_count, L.h_snap.p + i, L.stream)) != hipSuccess) return e;
+            e = hipEventRecord(L.sev[i], L.stream);
+        }
+        return e;
+    };
+    cof_result *res = nullptr;
+    if (pipe) {
+        const int rc = collect_ranges(L, qs, gbase, nlaunch, cap, end_bit, enqueue, res, err);
+        if (rc) {
+            (void)hipStreamSynchronize(L.stream);
+            (void)hipStreamSynchronize(L.stream2);
+            return rc;
+        }
+    } else {
+        for (size_t i = 0; i < nlaunch; i++)
+            HIP_TRY(enqueue(i), i < ktab.size() ? "fused launch" : "bulge launch");
     }
-    HIP_TRY(hipEventRecord(L.lev[nlaunch], L.stream), "hipEventRecord");
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
     HIP_TRY(hipMemcpyAsync(raw.data(), d_count, raw.size() * sizeof(unsigned long long), hipMemcpyDeviceToHost,
@@ -539,13 +744,19 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_comparer_ms = L.stats.t_fused_ms;
     L.stats.t_finder_ms = 0;
     L.launches.assign(nlaunch, LaunchRec());
+    double launch_ms = 0;
     for (size_t i = 0; i < nlaunch; i++) {
         LaunchRec &lr = L.launches[i];
         lr.kernel = kernels[i];
         lr.sig = sig[i];
         lr.variant = variant[i];
-        lr.ms = ms_between(L.lev[i], L.lev[i + 1]);
+        lr.ms = ms_between(L.lev[2 * i], L.lev[2 * i + 1]);
         lr.candidates = cnt[2 + i];
+        launch_ms += lr.ms;
+    }
+    if (pipe) {  // the per-launch sorts between the launches count as collection (t_sort_ms)
+        L.stats.t_fused_ms = launch_ms;
+        L.stats.t_comparer_ms = launch_ms;
     }
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
     for (size_t g = 0; g < ktab.size(); g++) {
@@ -575,6 +786,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.n_loci = 0;
     L.stats.n_finder_positions = 0;
     nhits = cnt[0];
+    if (res && nhits <= cap) *pres = res;
+    else delete res;
     return COF_OK;
 }
 
@@ -584,6 +797,8 @@ hipError_t Lane::init() {
     hipError_t e = prepare_kernels();
     if (e != hipSuccess) return e;
     if ((e = hipStreamCreate(&stream)) != hipSuccess) return e;
+    if ((e = hipStreamCreate(&stream2)) != hipSuccess) return e;
+    if (const char *v = getenv("COF_PIPELINE")) pipeline = strcmp(v, "0") != 0;
     for (auto &x : ev)
         if ((e = hipEventCreate(&x)) != hipSuccess) return e;
     return hipSuccess;
@@ -617,6 +832,14 @@ void Lane::release() {
     h_ms.release();
     for (hipEvent_t x : cev) (void)hipEventDestroy(x);
     cev.clear();
+    if (stream2) (void)hipStreamSynchronize(stream2);
+    for (hipEvent_t x : sev) (void)hipEventDestroy(x);
+    sev.clear();
+    for (hipEvent_t x : cev2) (void)hipEventDestroy(x);
+    cev2.clear();
+    h_snap.release();
+    if (stream2) (void)hipStreamDestroy(stream2);
+    stream2 = nullptr;
     d_count.release();
     d_loci.release();
     d_flags.release();
@@ -653,10 +876,13 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     }
     uint64_t nhits = 0;
     double tune_ms = 0;
+    const bool pipe = mode == COF_MODE_BATCHED && L.pipeline;
+    cof_result *pres = nullptr;
     for (int attempt = 0; attempt < 4; attempt++) {
         HIP_TRY(hipMemsetAsync(L.d_count.p, 0, L.d_count.n * sizeof(unsigned long long), L.stream), "hipMemsetAsync");
         int rc = mode == COF_MODE_PLAIN ? search_plain(L, dg, slab_positions, qs, gbase, nhits, err)
-                                        : search_batched(L, dg, qs, gbase, tuner, nhits, err);
+                                        : search_batched(L, dg, qs, gbase, tuner, nhits, err, nullptr,
+                                                         pipe ? &pres : nullptr);
         if (rc) return rc;
         tune_ms += L.stats.t_tune_ms;
         if (nhits <= L.d_hkey.n) break;
@@ -665,6 +891,14 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
         HIP_TRY(L.d_hmm.reserve(want), "hipMalloc(hits)");
     }
     if (nhits > L.d_hkey.n) { err = "Hit buffer overflow."; return COF_E_NOMEM; }
+    if (pres) {  // collected launch by launch
+        L.stats.t_total_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+        L.stats.n_hits = nhits;
+        L.stats.t_tune_ms = tune_ms;
+        L.stats.genome_len = dg.total;
+        *out = pres;
+        return COF_OK;
+    }
     const auto t_kern = std::chrono::steady_clock::now();
     HIP_TRY(hipEventRecord(L.ev[2], L.stream), "hipEventRecord");
     // sort on the device by (global key, start, strand), copy to pinned memory
@@ -777,11 +1011,13 @@ int cof::run_search(Lane &L, const DevGenome &dg, uint64_t slab_positions, const
     }
     L.stats.t_decode_ms = t_decode;
     r->queries = qs;
-    r->qbegin.assign(qs.size() + 1, 0);
-    for (size_t qi = 0; qi <= qs.size(); qi++)  // hits are sorted by query
-        r->qbegin[qi] = std::partition_point(r->hits.begin(), r->hits.end(),
-                                             [qi](const cof_hit &x) { return x.query < qi; }) -
-                        r->hits.begin();
+    r->qbegin.assign(qs.size(), 0);
+    r->qend.assign(qs.size(), 0);
+    for (size_t qi = 0; qi < qs.size(); qi++) {  // hits are sorted by query
+        auto by_q = [](const cof_hit &x, size_t q) { return x.query < q; };
+        r->qbegin[qi] = std::lower_bound(r->hits.begin(), r->hits.end(), qi, by_q) - r->hits.begin();
+        r->qend[qi] = std::lower_bound(r->hits.begin(), r->hits.end(), qi + 1, by_q) - r->hits.begin();
+    }
     const auto t1 = std::chrono::steady_clock::now();
     L.stats.t_total_ms = std::chrono::duration<double, std::milli>(t1 - t0).count();
     L.stats.t_collect_ms = std::chrono::duration<double, std::milli>(t1 - t_kern).count();

@@ -64,9 +64,11 @@ struct UninitAlloc : std::allocator<T> {
 }  // namespace cof
 
 struct cof_result {
-    std::vector<cof_hit, cof::UninitAlloc<cof_hit>> hits;  // sorted (query, key, pos, strand)
+    // hits of query i: [qbegin[i], qend[i]), sorted by (key, pos, strand); the
+    // queries' runs lie in launch order, not necessarily in query order
+    std::vector<cof_hit, cof::UninitAlloc<cof_hit>> hits;
     std::vector<std::shared_ptr<cof::Query>> queries;
-    std::vector<size_t> qbegin;                     // hits of query i: [qbegin[i], qbegin[i+1])
+    std::vector<size_t> qbegin, qend;
     uint64_t genome_serial = 0;                     // the genome searched (cof_ctx::genome_serial)
 };
 
@@ -126,6 +128,7 @@ hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t
 // Packs n sorted hits (genome below 2^32 positions) into pos[i] (window
 // start) and ms[i] (mismatches << 1 | strand), and koff[0..nkeys] into the
 // first hit of each global key (nkeys: the search's key count).
+hipError_t snap_count(const unsigned long long *count, unsigned long long *dst, hipStream_t s);
 hipError_t pack_hits(const uint64_t *hkey, const uint16_t *hmm, uint64_t n, uint32_t nkeys, uint32_t *pos, uint8_t *ms,
                      uint32_t *koff, hipStream_t s);
 // Kernel instance of the calling thread's last batched launch, as profilers
@@ -209,6 +212,13 @@ struct BatchPlan {
 // may run concurrently over the same resident genome.
 struct Lane {
     hipStream_t stream = nullptr;
+    // collection stream: the sort and copy of each batched launch's hits run
+    // there while the next launches run on `stream`
+    hipStream_t stream2 = nullptr;
+    HostBuf<unsigned long long> h_snap;  // hit counter after each batched launch
+    std::vector<hipEvent_t> sev;         // after each launch's counter snapshot
+    std::vector<hipEvent_t> cev2;        // per launch: before sort, after sort, after copy
+    bool pipeline = true;                // per-launch collection (COF_PIPELINE=0: after the last launch)
     DevBuf<uint64_t> d_hkey, d_hkey2;  // hit sort keys (emitted, sorted)
     DevBuf<uint16_t> d_hmm, d_hmm2;    // hit mismatch counts (emitted, sorted)
     DevBuf<uint8_t> d_sort_tmp;

@@ -18,6 +18,17 @@ hipError_t sort_hits(void *tmp, size_t *tmp_bytes, const uint64_t *kin, uint64_t
     return rocprim::radix_sort_pairs(tmp, *tmp_bytes, kin, kout, vin, vout, (size_t)n, 0u, end_bit, s);
 }
 
+// *dst = the hit counter, ordered after the launches before it on the stream
+// (dst may be pinned host memory)
+__global__ __launch_bounds__(64) void snap_count_kernel(const unsigned long long *count, unsigned long long *dst) {
+    if (threadIdx.x == 0) dst[0] = __hip_atomic_load(count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+hipError_t snap_count(const unsigned long long *count, unsigned long long *dst, hipStream_t s) {
+    hipLaunchKernelGGL(snap_count_kernel, dim3(1), dim3(64), 0, s, count, dst);
+    return hipGetLastError();
+}
+
 __global__ __launch_bounds__(256) void fill_u32_kernel(uint32_t *p, uint64_t n, uint32_t v) {
     const uint64_t i = (uint64_t)blockIdx.x * 256 + threadIdx.x;
     if (i < n) p[i] = v;

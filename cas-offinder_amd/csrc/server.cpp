@@ -40,9 +40,9 @@ struct Request {
     std::string err;
 };
 
-// Device bytes per hit in a lane: emitted and sorted sort keys (8 + 8),
-// mismatch counts (2 + 2) and the packed copy (4 + 1).
-constexpr uint64_t kLaneBytesPerHit = 25;
+// Device bytes per hit in a lane: emitted and sorted sort keys (8 + 8) and
+// mismatch counts (2 + 2).
+constexpr uint64_t kLaneBytesPerHit = 20;
 constexpr uint64_t kMinLaneHits = 1ull << 20, kMaxLaneHits = 1ull << 26;
 
 // Table bytes of a query: its key table, or for bulge queries a record and a
@@ -130,17 +130,20 @@ void cof_server::worker(Lane *L) {
                 p->genome_serial = ctx->genome_serial;  // reloads are refused while the server runs
                 const size_t nqi = batch[i]->qs.size();
                 p->queries = batch[i]->qs;
-                p->qbegin.assign(nqi + 1, 0);
-                const size_t h0 = res->qbegin[q0];
-                p->hits.resize(res->qbegin[q0 + nqi] - h0);
+                p->qbegin.assign(nqi, 0);
+                p->qend.assign(nqi, 0);
+                size_t total = 0;
+                for (size_t j = 0; j < nqi; j++) total += res->qend[q0 + j] - res->qbegin[q0 + j];
+                p->hits.resize(total);
+                size_t at = 0;
                 for (size_t j = 0; j < nqi; j++) {
-                    p->qbegin[j] = res->qbegin[q0 + j] - h0;
-                    for (size_t h = res->qbegin[q0 + j]; h < res->qbegin[q0 + j + 1]; h++) {
-                        p->hits[h - h0] = res->hits[h];
-                        p->hits[h - h0].query = (uint8_t)j;
+                    p->qbegin[j] = at;
+                    for (size_t h = res->qbegin[q0 + j]; h < res->qend[q0 + j]; h++, at++) {
+                        p->hits[at] = res->hits[h];
+                        p->hits[at].query = (uint8_t)j;
                     }
+                    p->qend[j] = at;
                 }
-                p->qbegin[nqi] = p->hits.size();
                 parts[i] = p;
                 q0 += nqi;
             }
@@ -194,8 +197,7 @@ int cof_server_start(cof_ctx *ctx, const cof_server_cfg *cfg, cof_server **out) 
         const uint64_t n = s->lane_hits;
         if (L->init() != hipSuccess || L->d_hkey.reserve(n) != hipSuccess || L->d_hmm.reserve(n) != hipSuccess ||
             L->d_hkey2.reserve(n) != hipSuccess || L->d_hmm2.reserve(n) != hipSuccess ||
-            L->d_pos32.reserve(n) != hipSuccess || L->d_ms.reserve(n) != hipSuccess ||
-            L->h_pos32.reserve(n) != hipSuccess || L->h_ms.reserve(n) != hipSuccess) {
+            L->h_hkey.reserve(n) != hipSuccess || L->h_hmm.reserve(n) != hipSuccess) {
             for (auto &x : s->lanes) x->release();
             L->release();
             delete s;

@@ -249,7 +249,7 @@ int cof_server_submit(cof_server *s, cof_query *const *queries, int n_queries, u
 int cof_server_poll(cof_server *s, uint64_t ticket, int wait, cof_result **out, double *latency_ms);
 int cof_server_stop(cof_server *s);
 /* What the server did so far: each lane's hit capacity (its share of
- * mem_budget_bytes at 25 device bytes per hit, 1 M to 64 M hits), served
+ * mem_budget_bytes at 20 device bytes per hit, 1 M to 64 M hits), served
  * batches / requests / queries / hits, batches whose hits outgrew the lane
  * (grown and rerun), admissions that waited for running batches, the peak
  * estimate of in-flight bytes, and the running average of hits per compare

@@ -2,10 +2,12 @@
 GPU work and synchronizations between replays, gives the records of eager
 launches (a context with COF_GRAPH=0)."""
 import os
+import random
 
 import pytest
 
 import casoff
+import helpers as H
 import workloads
 
 pytestmark = pytest.mark.gpu
@@ -96,7 +98,6 @@ def test_plain_search_between_batched_searches(tmp_path):
     own tables again instead of replaying the captured graph over the plain
     path's.  Bulge groups are in the set: their kernels take variant-block
     offsets from the tables.  Every search equals the oracle."""
-    import helpers as H
     gp = str(tmp_path / "g.fa")
     casoff.synth_write(casoff.synth_spec(seed=23, n_contigs=4, total_len=1_200_000, n_frac=0.01), gp, 0)
     texts = [t for _, t in workloads.group_inputs(workloads.mixed_guides(6, 40, 3, 6, 0.5, 1), gp)]
@@ -120,11 +121,28 @@ def test_plain_search_between_batched_searches(tmp_path):
         ctx.close()
 
 
+def _ctx_with_env(env):
+    """A context opened with the variables of `env` set (lanes read them at open)."""
+    old = {k: os.environ.get(k) for k in env}
+    os.environ.update(env)
+    try:
+        return casoff.Context(0)
+    finally:
+        for k, v in old.items():
+            if v is None:
+                os.environ.pop(k)
+            else:
+                os.environ[k] = v
+
+
 def test_hit_copy_forms_agree(tmp_path):
-    """Hits cross to the host packed (5 bytes and a per-key first-hit table)
-    below 2^32 genome positions, and as 10-byte records otherwise
-    (COF_HIT_PACK=0 forces that form here): both decode to the same records,
-    over several copy chunks (a workload of more than 4 M hits)."""
+    """Batched hits are collected launch by launch (sorted and copied on a
+    second stream while later launches run) or after the last launch
+    (COF_PIPELINE=0); there they cross to the host packed (5 bytes and a
+    per-key first-hit table) below 2^32 genome positions, and as 10-byte
+    records otherwise (COF_HIT_PACK=0 forces that form here).  Every form
+    decodes to the same records, over several copy chunks and a hit buffer
+    that overflows on the first pass (a workload of more than 4 M hits)."""
     gp = str(tmp_path / "g.fa")
     casoff.synth_write(casoff.synth_spec(seed=24, n_contigs=3, total_len=12_000_000, n_frac=0.01), gp, 0)
     # all-N pattern, 8 comparable positions at threshold 3: ~6% of the windows
@@ -132,23 +150,53 @@ def test_hit_copy_forms_agree(tmp_path):
     guides = ["ACGTACGT", "TTGCACGT", "CCGTACGA", "GATTACAG", "AACCGGTT", "TGCATGCA", "CATGACTA", "GGACTTCA"]
     text = gp + "\nNNNNNNNNNNNNNNNN\n" + "".join("%sNNNNNNNN 3\n" % g for g in guides)
     got = {}
-    for pack in ("1", "0"):
-        old = os.environ.get("COF_HIT_PACK")
-        os.environ["COF_HIT_PACK"] = pack
-        try:
-            c = casoff.Context(0)
-        finally:
-            if old is None:
-                os.environ.pop("COF_HIT_PACK")
-            else:
-                os.environ["COF_HIT_PACK"] = old
+    for pack, pipe in (("1", "1"), ("1", "0"), ("0", "0")):
+        c = _ctx_with_env({"COF_HIT_PACK": pack, "COF_PIPELINE": pipe})
         try:
             c.load(gp)
             q = c.query(text)
-            got[pack] = [c.search([q], m).records() for m in (casoff.MODE_BATCHED, casoff.MODE_PLAIN)]
+            got[pack + pipe] = [c.search([q], m).records() for m in (casoff.MODE_BATCHED, casoff.MODE_PLAIN)]
         finally:
             c.close()
-    assert len(got["1"][0]) > (1 << 22) + 1000
-    for a, b in zip(got["1"], got["0"]):
-        assert len(a) == len(b) and (a == b).all()
-    assert (got["1"][0] == got["1"][1]).all()
+    ref = got["11"]
+    assert len(ref[0]) > (1 << 22) + 1000
+    for form in ("10", "00"):
+        for a, b in zip(ref, got[form]):
+            assert len(a) == len(b) and (a == b).all(), form
+    assert (ref[0] == ref[1]).all()
+
+
+def test_launch_order_differs_from_query_order(tmp_path):
+    """Key groups launch before bulge groups and queries of one pattern share
+    a launch, so the launch-by-launch collection stores query runs out of
+    query order: records, per-query formatting and the summary must still
+    come out per query as each query searched alone on the plain path."""
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=25, n_contigs=4, total_len=3_000_000, n_frac=0.01), gp, 0)
+    r = random.Random(25)
+    rand = lambda n: "".join(r.choice("ACGT") for _ in range(n))
+    texts = [gp + "\nNNNNNNNNNNNNNNNNNNNNNGG 1 1\n" + "".join("%sNNN 4\n" % rand(20) for _ in range(3)),
+             gp + "\nNNNNNNNNNNNNNNNNNNNNNGG\n" + "".join("%sNNN 5\n" % rand(20) for _ in range(5)),
+             gp + "\nNNNNNNNNNNNNNNNNNNNNNRG\n" + "".join("%sNNN 4\n" % rand(20) for _ in range(4)),
+             gp + "\nNNNNNNNNNNNNNNNNNNNNNGG\n" + "".join("%sNNN 4\n" % rand(20) for _ in range(6))]
+    ctx = casoff.Context(0)
+    try:
+        ctx.load(gp)
+        qs = [ctx.query(t) for t in texts]
+        res = ctx.search(qs, casoff.MODE_BATCHED)
+        assert ctx.stats()["n_launches"] == 3  # NGG keys (queries 1 and 3), NRG keys, the NGG bulge group
+        alone = [ctx.search([q], casoff.MODE_PLAIN) for q in qs]
+        recs = res.records()
+        assert len(recs) == sum(len(a.records()) for a in alone) > 0
+        assert (recs["query"][1:] >= recs["query"][:-1]).all()  # query order
+        at = 0
+        for i, a in enumerate(alone):
+            ar = a.records()
+            got = recs[at:at + len(ar)]
+            assert (got["query"] == i).all()
+            for f in ("pos", "key", "mm", "strand"):
+                assert (got[f] == ar[f]).all(), (i, f)
+            at += len(ar)
+            assert H.record_lines(res.format(i)) == H.record_lines(a.format(0)), i
+    finally:
+        ctx.close()
