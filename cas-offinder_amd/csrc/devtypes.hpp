@@ -136,9 +136,17 @@ struct SeedDesc {
 // parts - t of the key's ranges exactly; the kernel counts only the records
 // that match at least `minex` ranges (parts >= largest threshold + minex;
 // minex 3 below: a count per word).
-// Layout (words, copied to LDS as is): rows (row i at i * nk; row zrow is all
-// zero: the unused parts up to the kernel's part count address it with span
-// 0), then at recw the records [klo, khi, bias, key index].
+// Layout (words, copied to LDS as is): rows, then at recw the records [klo,
+// khi, bias, key index].  Row zrow is all zero: the unused parts up to the
+// kernel's part count address it with span 0.  Rows of nk < 4 words lie one
+// after another (row i at i * nk); wider rows are stored chunk-major, words
+// [4c, 4c + 4) of row i at c * mask_plane(zrow) + 4 * i, so that the 16-byte
+// chunk reads of lanes at random rows spread over all LDS banks (rows of 8,
+// 16 or 24 consecutive words would start on a few banks only).
+constexpr uint32_t mask_plane(uint32_t zrow) { return (zrow + 1) * 4; }
+constexpr uint32_t mask_word(uint32_t nk, uint32_t zrow, uint32_t row, uint32_t w) {
+    return nk < 4 ? row * nk + w : (w / 4) * mask_plane(zrow) + row * 4 + w % 4;
+}
 constexpr int kMaskMaxWords = 24;
 constexpr int kMaskMaxSpan = 4;
 constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;

@@ -58,6 +58,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
     const uint32_t *s_rows = s_idx;
     const uint4 *s_rec = reinterpret_cast<const uint4 *>(s_idx + md.recw);
     const uint32_t care = md.care;
+    const uint32_t plane = mask_plane(md.zrow);  // words between chunk planes (rows of 4 or more words)
     const uint32_t fwd_bit = a.cbase + 32 * kHaloL, rev_bit = a.pd.P - 32 - a.cbase + 32 * kHaloL;
     tile_candidates<1, false, LC, G>(
         a.g, a.pd, a.limit, a.tile_stride, a.cand_count, g_lo, g_hi, g_ot, s_list[grp], s_wsum[grp],
@@ -74,6 +75,9 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
             // may select keys it does not match: extra survivors that the
             // exact count below rejects.  No hit is lost: the ranges that a
             // hit's key matches exactly hold no N.
+            // row pointers: words 0-3 of the row (chunk c of a row of 4 or
+            // more words at c * plane further, chunk-major layout)
+            constexpr uint32_t RS = NK < 4 ? NK : 4;
             const uint32_t *row[NP];
 #pragma unroll
             for (int p = 0; p < NP; p++) {
@@ -81,7 +85,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
                 const uint32_t a0 = f & 0xFF, span = (f >> 8) & 0xFF;
                 const uint32_t code =
                     __builtin_amdgcn_ubfe(l, a0, span) | (__builtin_amdgcn_ubfe(h, a0, span) << span);
-                row[p] = s_rows + (f >> 16) * NK + code * NK;  // the range's first row is uniform
+                row[p] = s_rows + (f >> 16) * RS + code * RS;  // the range's first row is uniform
             }
             auto count = [&](uint32_t r) {  // exact count of surviving record r
                 const uint4 k = s_rec[r];
@@ -106,7 +110,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
                 for (int c = 0; c < NK / 4; c++) {
                     uint32_t m[NP][4];
 #pragma unroll
-                    for (int p = 0; p < NP; p++) load_words<4>(row[p] + 4 * c, m[p]);
+                    for (int p = 0; p < NP; p++) load_words<4>(row[p] + c * plane, m[p]);
                     if constexpr (MINEX == 3) {
 #pragma unroll
                         for (int w = 0; w < 4; w++) {
@@ -130,7 +134,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
             for (int c = 0; c < NK / CH; c++) {
                 uint32_t m[NP][CH];
 #pragma unroll
-                for (int p = 0; p < NP; p++) load_words<CH>(row[p] + CH * c, m[p]);
+                for (int p = 0; p < NP; p++) load_words<CH>(row[p] + c * plane, m[p]);
                 uint32_t any[CH], two[CH], sel[CH];
                 if constexpr (MINEX == 3) {  // per-word counts
 #pragma unroll

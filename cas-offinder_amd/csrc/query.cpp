@@ -475,7 +475,8 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
             const int a = (int)(d.first[p] & 0xFF), span = (int)((d.first[p] >> 8) & 0xFF);
             uint32_t cs[1u << (2 * kMaskMaxSpan)];
             const int nc = range_codes(k, (size_t)(t.cbase + a), span, cs, 1 << (2 * kMaskMaxSpan));
-            for (int j = 0; j < nc; j++) s.words[(size_t)(rbase[p] + cs[j]) * d.nk + r / 32] |= 1u << (r % 32);
+            for (int j = 0; j < nc; j++)
+                s.words[mask_word(d.nk, d.zrow, rbase[p] + cs[j], (uint32_t)(r / 32))] |= 1u << (r % 32);
         }
         // the key table's record of key kr (single class: records follow the header in key order)
         for (int w = 0; w < 4; w++) s.words[d.recw + 4 * r + w] = t.words[kClassHdr + 4 * kr + w];
