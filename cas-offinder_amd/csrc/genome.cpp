@@ -3,8 +3,10 @@
 
 #include <dirent.h>
 #include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <climits>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -208,12 +210,26 @@ uint64_t HostGenome::n_contigs() const {
 
 // ------------------------------------------------------------------ readers
 
+// The file a reader opens for `path`: when `path` is a symbolic link, the
+// link's raw target text (read_fasta.cpp:17-22, read_twobit.cpp:58-63), so a
+// relative target resolves against the working directory, not the link's
+// directory; otherwise `path` itself.
+static FILE *open_genome_file(const std::string &path) {
+    char buf[PATH_MAX + 1];
+    const ssize_t n = readlink(path.c_str(), buf, PATH_MAX);
+    if (n >= 0) {
+        buf[n] = 0;
+        return fopen(buf, "rb");
+    }
+    return fopen(path.c_str(), "rb");
+}
+
 // FASTA as read_fasta.cpp:11-57 holds it: lines split on '\n', one trailing
 // '\r' dropped, empty lines skipped, '>' lines start a record (name = rest of
 // the line, ';' before every record but the first), other lines uppercased
 // (C-locale toupper) and appended.
 int load_fasta(const std::string &path, HostGenome &g, std::string &err) {
-    FILE *f = fopen(path.c_str(), "rb");
+    FILE *f = open_genome_file(path);
     if (!f) return 1;
     int c0 = fgetc(f);
     if (c0 != '>') { fclose(f); return 1; }
@@ -315,7 +331,7 @@ static bool rd32(FILE *f, uint32_t &v) { return fread(&v, 4, 1, f) == 1; }
 // order after the index, bases T/C/A/G MSB-first, N blocks as 'N', mask blocks
 // ignored, ';' between records.
 int load_twobit(const std::string &path, HostGenome &g, std::string &err) {
-    FILE *f = fopen(path.c_str(), "rb");
+    FILE *f = open_genome_file(path);
     if (!f) return 1;
     uint32_t magic, ver, cnt, tmp;
     if (!rd32(f, magic) || magic != 0x1A412743u) { fclose(f); return 1; }

@@ -47,6 +47,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <sys/stat.h>
+#include <unistd.h>
+#include <limits.h>
 #include <time.h>
 #ifdef _OPENMP
 #include <omp.h>
@@ -171,12 +173,22 @@ void orc_genome_free(orc_genome *g) {
     memset(g, 0, sizeof(*g));
 }
 
+/* read_fasta.cpp:17-22, read_twobit.cpp:58-63: a symbolic link is opened
+ * through its raw target text (relative targets resolve against the working
+ * directory); any other path as given. */
+static FILE *open_genome(const char *path) {
+    char buf[PATH_MAX + 1];
+    memset(buf, 0, sizeof(buf));
+    ssize_t n = readlink(path, buf, PATH_MAX);
+    return fopen(n >= 0 ? buf : path, "rb");
+}
+
 /* read_fasta.cpp:11-57.  One string per file, records joined by ';', each
  * sequence line uppercased, one trailing '\r' dropped, empty lines skipped,
  * name = header line after '>'.  Returns 1 when the first byte is not '>'. */
 int orc_read_fasta(const char *path, orc_genome *g) {
     g_reset(g);
-    FILE *f = fopen(path, "rb");
+    FILE *f = open_genome(path);
     if (!f) return 1;
     int c0 = fgetc(f);
     if (c0 != '>') { fclose(f); return 1; }
@@ -207,7 +219,7 @@ static int rd_u32(FILE *f, uint32_t *v) { return fread(v, 4, 1, f) == 1 ? 0 : -1
  * MSB-first, N blocks set to 'N', mask blocks ignored, ';' between records. */
 int orc_read_twobit(const char *path, orc_genome *g) {
     g_reset(g);
-    FILE *f = fopen(path, "rb");
+    FILE *f = open_genome(path);
     if (!f) return 1;
     uint32_t magic = 0, ver = 1, cnt = 0, tmp;
     if (rd_u32(f, &magic) || magic != 440477507u) { fclose(f); return 1; }   /* :66 */

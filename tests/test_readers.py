@@ -179,3 +179,28 @@ def test_large_2bit_and_fasta_agree(tmp_path):
     a = check_all(fa)
     b = check_all(tb)
     assert a == b
+
+
+def test_symlink_opens_raw_target_against_cwd(tmp_path, monkeypatch):
+    """A symbolic link is opened through its raw target text
+    (read_fasta.cpp:17-22, read_twobit.cpp:58-63): a relative target resolves
+    against the working directory, not the link's directory.  The file the
+    link's own directory would give holds another genome, so reading it
+    would fail the comparison."""
+    cwd, gdir = tmp_path / "cwd", tmp_path / "gdir"
+    (cwd / "data").mkdir(parents=True)
+    (gdir / "data").mkdir(parents=True)
+    H.write_fasta(str(cwd / "data" / "real.fa"), [("from_cwd", "ACGTNNACGT")])
+    H.write_fasta(str(gdir / "data" / "real.fa"), [("from_link_dir", "TTTTTTTT")])
+    _write_2bit(str(tmp_path / "abs.2bit"), [("abs2bit", "GGGACCCT")])
+    os.symlink("data/real.fa", str(gdir / "rel.fa"))
+    os.symlink(str(tmp_path / "abs.2bit"), str(gdir / "abs.2bit"))
+    monkeypatch.chdir(str(cwd))
+    o = check_all(str(gdir / "rel.fa"))
+    assert o[1] == [(0, "from_cwd")] and o[2] == "ACGTNNACGT"
+    o = check_all(str(gdir / "abs.2bit"))
+    assert o[1] == [(0, "abs2bit")] and o[2] == "GGGACCCT"
+    # a directory load reads the links (DT_LNK entries, main.cpp:141-143) the same way
+    c = casoff.Context(-1)
+    assert c.load(str(gdir)) == 2
+    assert sorted(nm for nm, _, _, _ in c.contigs()) == ["abs2bit", "from_cwd"]
