@@ -361,9 +361,13 @@ def main():
     ctx.synth(workloads.genome_3g(total_len=args.genome_len))
     t_load = time.perf_counter() - t0
     total, nfiles, ncontigs = ctx.info()
-    log("genome: %d positions, %d contigs, generated+uploaded in %.1f s" % (total, ncontigs, t_load))
+    up_ms = ctx.stats()["t_upload_ms"]
+    up_bytes = 3 * 4 * ((total + 31) // 32)
+    log("genome: %d positions, %d contigs, generated+uploaded in %.1f s (upload %.1f ms, %.1f GB/s)"
+        % (total, ncontigs, t_load, up_ms, up_bytes / up_ms / 1e6))
     texts, main_res = run_workload(ctx, args.workload, args.steps, args.warmup, args.plain_steps, not args.no_plain)
-    main_res["autotune"]["variants"] = ctx.tuner_info()
+    used = {rec["sig"] for rec in main_res.get("launches", [])}
+    main_res["autotune"]["variants"] = {k: v for k, v in ctx.tuner_info().items() if k in used}
     out = {
         "metric": METRIC,
         "value": main_res["value"],
@@ -389,6 +393,8 @@ def main():
         main_res.pop(k)
     out.update(main_res)
     out["genome_generate_upload_s"] = round(t_load, 2)
+    out["genome_upload"] = {"ms": round(up_ms, 1), "plane_bytes": up_bytes, "GB_per_s": round(up_bytes / up_ms / 1e6, 1),
+                            "path": "pageable planes -> 2 x 64 MiB pinned staging -> HBM (hipMemcpyAsync)"}
     extra = [w for w in args.extra_workloads.split(",") if w and w != args.workload]
     if extra:
         out["workloads"] = {}

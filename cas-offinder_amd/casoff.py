@@ -56,6 +56,7 @@ class Stats(C.Structure):
         ("t_sort_ms", C.c_double),
         ("t_copy_ms", C.c_double),
         ("t_decode_ms", C.c_double),
+        ("t_upload_ms", C.c_double),
     ]
 
     def as_dict(self):

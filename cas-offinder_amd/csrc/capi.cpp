@@ -281,6 +281,7 @@ int cof_search(cof_ctx *c, cof_query *const *queries, int n_queries, int mode, c
 int cof_stats_get(const cof_ctx *c, cof_stats *out) {
     if (!c || !out) return COF_E_INVALID;
     *out = c->lane.stats;
+    out->t_upload_ms = c->t_upload_ms;
     return COF_OK;
 }
 

@@ -290,9 +290,10 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
 // hits end.  The host keeps one launch ahead of the device: once launch i is
 // done (launch i + 1 already queued), it queues the radix sort of launch i's
 // range behind launch i + 1 on the lane's stream (the sort gets the whole
-// GPU, between launches), then the range's copy on the second stream, and
-// host threads decode copied ranges into the result at the same offsets
-// while later launches run.  Every query's keys go to exactly one launch and
+// GPU, between launches: on the second stream beside a search kernel it ran
+// five times slower and only moved its time to the end of the search), then
+// the range's copy on the second stream, and host threads decode copied
+// ranges into the result at the same offsets while later launches run.  Every query's keys go to exactly one launch and
 // every key of a launch belongs to its queries, so each query's hits are one
 // sorted run inside its launch's range.  enqueue(i) queues launch i and its
 // counter snapshot (event L.sev[i]).
@@ -313,6 +314,8 @@ static int collect_ranges(Lane &L, const std::vector<std::shared_ptr<Query>> &qs
         const auto td = std::chrono::steady_clock::now();
         const uint64_t *hk = L.h_hkey.p;
         const uint16_t *hm = L.h_hmm.p;
+        // two host threads stay free: the one queueing launches and sorts,
+        // and this decoder's own
         parallel_for(e - b, 1 << 16, [&](uint64_t x0, uint64_t x1) {
             x0 += b, x1 += b;
             size_t q = (size_t)(std::upper_bound(gbase.begin(), gbase.end() - 1, (uint32_t)(hk[x0] >> kHitKeyShift)) -
@@ -328,7 +331,7 @@ static int collect_ranges(Lane &L, const std::vector<std::shared_ptr<Query>> &qs
                 h.mm = hm[x];
                 h.query = (uint8_t)q;
             }
-        });
+        }, std::max(1, n_host_threads() - 2));
         for (size_t q = 0; q < nq; q++) {
             const uint64_t *a = std::lower_bound(hk + b, hk + e, (uint64_t)gbase[q] << kHitKeyShift);
             const uint64_t *z = std::lower_bound(a, hk + e, (uint64_t)gbase[q + 1] << kHitKeyShift);
@@ -1048,16 +1051,60 @@ int cof::run_search_noalloc(Lane &L, const DevGenome &dg, const std::vector<std:
     return COF_OK;
 }
 
+// Host-to-device copy of a large pageable array through two pinned staging
+// buffers: host threads fill one buffer while the DMA engine reads the
+// other, so the transfer runs at the pinned-memory rate and overlaps the
+// host-side copy.  done[b] marks the end of the last copy out of buffer b.
+static hipError_t upload_staged(void *dst, const void *src, uint64_t bytes, HostBuf<uint8_t> (&stage)[2],
+                                hipEvent_t (&done)[2], hipStream_t s) {
+    const uint64_t chunk = stage[0].n;
+    int b = 0;
+    for (uint64_t off = 0; off < bytes; off += chunk, b ^= 1) {
+        const uint64_t n = std::min(chunk, bytes - off);
+        hipError_t e = hipEventSynchronize(done[b]);
+        if (e != hipSuccess) return e;
+        uint8_t *st = stage[b].p;
+        const uint8_t *from = static_cast<const uint8_t *>(src) + off;
+        parallel_for(n, 1 << 22, [&](uint64_t x0, uint64_t x1) { memcpy(st + x0, from + x0, x1 - x0); });
+        if ((e = hipMemcpyAsync(static_cast<uint8_t *>(dst) + off, st, n, hipMemcpyHostToDevice, s)) != hipSuccess)
+            return e;
+        if ((e = hipEventRecord(done[b], s)) != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
 int cof_ctx::upload_genome() {
     genome_ready = false;
+    const auto tu0 = std::chrono::steady_clock::now();
     const HostGenome &g = genome;
     const uint64_t nw = g.lo.size();  // includes the zero padding words
     HIP_TRY(d_lo.reserve(nw), "hipMalloc(genome lo)");
     HIP_TRY(d_hi.reserve(nw), "hipMalloc(genome hi)");
     HIP_TRY(d_ot.reserve(nw), "hipMalloc(genome ot)");
-    HIP_TRY(hipMemcpy(d_lo.p, g.lo.data(), nw * 4, hipMemcpyHostToDevice), "hipMemcpy(genome)");
-    HIP_TRY(hipMemcpy(d_hi.p, g.hi.data(), nw * 4, hipMemcpyHostToDevice), "hipMemcpy(genome)");
-    HIP_TRY(hipMemcpy(d_ot.p, g.ot.data(), nw * 4, hipMemcpyHostToDevice), "hipMemcpy(genome)");
+    {
+        // the three bit planes through 2 x 64 MiB of pinned staging memory
+        HostBuf<uint8_t> stage[2];
+        hipEvent_t done[2] = {nullptr, nullptr};
+        struct Free {
+            HostBuf<uint8_t> (&st)[2];
+            hipEvent_t (&ev)[2];
+            ~Free() {
+                for (auto &x : st) x.release();
+                for (auto &e : ev)
+                    if (e) (void)hipEventDestroy(e);
+            }
+        } free_on_exit{stage, done};
+        const uint64_t chunk = std::min<uint64_t>(64ull << 20, std::max<uint64_t>(nw * 4, 1));
+        for (int b = 0; b < 2; b++) {
+            HIP_TRY(stage[b].reserve(chunk), "hipHostMalloc(genome staging)");
+            HIP_TRY(hipEventCreate(&done[b]), "hipEventCreate");
+            HIP_TRY(hipEventRecord(done[b], lane.stream), "hipEventRecord");
+        }
+        HIP_TRY(upload_staged(d_lo.p, g.lo.data(), nw * 4, stage, done, lane.stream), "hipMemcpy(genome)");
+        HIP_TRY(upload_staged(d_hi.p, g.hi.data(), nw * 4, stage, done, lane.stream), "hipMemcpy(genome)");
+        HIP_TRY(upload_staged(d_ot.p, g.ot.data(), nw * 4, stage, done, lane.stream), "hipMemcpy(genome)");
+        HIP_TRY(hipStreamSynchronize(lane.stream), "hipMemcpy(genome)");
+    }
     std::vector<uint64_t> pz(g.poison);
     pz.push_back(~0ull);  // sentinel
     HIP_TRY(d_poison.reserve(pz.size()), "hipMalloc(poison)");
@@ -1085,6 +1132,7 @@ int cof_ctx::upload_genome() {
     dg.total = g.total;
     dg.nwords = g.nwords();
     genome_ready = true;
+    t_upload_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tu0).count();
     return COF_OK;
 }
 

@@ -346,6 +346,7 @@ struct cof_ctx {
     cof::Lane lane;                        // the context's own lane
     cof::Tuner tuner;
     uint64_t slab_positions = 1ull << 31;  // plain path window starts per finder launch
+    double t_upload_ms = 0;                // duration of the last upload_genome
 
     int fail(int code, const std::string &m) {
         err = m;

@@ -25,9 +25,10 @@ int n_host_threads() {
     return (int)std::min(n, 16u);
 }
 
-void parallel_for(uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn) {
+void parallel_for(uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn, int max_threads) {
     if (n == 0) return;
     uint64_t nt = (uint64_t)n_host_threads();
+    if (max_threads > 0) nt = std::min<uint64_t>(nt, (uint64_t)max_threads);
     if (grain == 0) grain = 1;
     nt = std::min<uint64_t>(nt, (n + grain - 1) / grain);
     if (nt <= 1) { fn(0, n); return; }

@@ -80,7 +80,10 @@ void synth_genome(const cof_synth_spec &spec, HostGenome &g);
 int synth_write(const cof_synth_spec &spec, const std::string &path, int format, std::string &err);
 
 // Parallel-for over [0, n) on std::threads.
-void parallel_for(uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn);
+// fn over [0, n) split into at most n_host_threads() (and max_threads, when
+// positive) ranges of at least `grain`, one thread each
+void parallel_for(uint64_t n, uint64_t grain, const std::function<void(uint64_t, uint64_t)> &fn,
+                  int max_threads = 0);
 int n_host_threads();
 
 }  // namespace cof

@@ -74,6 +74,8 @@ typedef struct {
     double t_sort_ms;       /* collection: device sort and packing of the hits (HIP events) */
     double t_copy_ms;       /* collection: device-to-host copies of the hits (HIP events) */
     double t_decode_ms;     /* collection: host decoding into cof_hit records (overlaps the copies) */
+    double t_upload_ms;     /* the context's last genome upload: the three bit planes through pinned
+                               staging buffers, plus the poison and tile tables (not per search) */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference
