@@ -149,6 +149,10 @@ constexpr uint32_t mask_word(uint32_t nk, uint32_t zrow, uint32_t row, uint32_t 
 }
 constexpr int kMaskMaxWords = 24;
 constexpr int kMaskMaxSpan = 4;
+// per-word counts (minex 3) of tables of kMaskWideWords or more words may use
+// up to kMaskMaxParts ranges; the others at most kSeedMaxParts
+constexpr int kMaskMaxParts = 12;
+constexpr int kMaskWideWords = 12;
 constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;
 // minex 3: thresholds differ across the table; records are ordered by
 // threshold and word w keeps the records that match at least need(w) =
@@ -157,7 +161,7 @@ constexpr uint32_t kMaskMaxIndexBytes = 40 * 1024;
 struct MaskDesc {
     uint32_t parts, nk, minex, nrec;
     uint32_t zrow, recw, nwords, care;  // nwords: rows and records, a multiple of 4
-    uint32_t first[kSeedMaxParts];  // bit of the range's first position | span << 8 | first row << 16
+    uint32_t first[kMaskMaxParts];  // bit of the range's first position | span << 8 | first row << 16
     uint32_t need[(kMaskMaxWords + 7) / 8];
 };
 

@@ -38,7 +38,6 @@ struct MaskArgs {
 // kMaskWideWords), whose LDS index allows no more waves anyway; with per-word
 // counts their rows are read and counted for all words first, so that the
 // reads of every chunk issue before the survivor loops.
-constexpr int kMaskWideWords = 12;
 template <int NK, int MINEX, int NP, int G>
 __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fused_mask_kernel(MaskArgs a) {
     constexpr int LC = G == 1 ? kSeedListCap : kListCap;
@@ -186,6 +185,20 @@ static hipError_t mask_attr() {
             if (e == hipSuccess)
                 e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
     }
+    if constexpr (MINEX == 3 && NK >= kMaskWideWords) {  // 10 and 12 rows
+        const void *k3[2] = {(const void *)fused_mask_kernel<NK, MINEX, 10, 3>,
+                             (const void *)fused_mask_kernel<NK, MINEX, 12, 3>};
+        for (const void *f : k3)
+            if (e == hipSuccess)
+                e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
+        if constexpr (NK >= kMaskWideWords) {
+            const void *k4[2] = {(const void *)fused_mask_kernel<NK, MINEX, 10, 2>,
+                                 (const void *)fused_mask_kernel<NK, MINEX, 12, 2>};
+            for (const void *f : k4)
+                if (e == hipSuccess)
+                    e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaskMaxIndexBytes);
+        }
+    }
     return e;
 }
 
@@ -237,17 +250,27 @@ static hipError_t mask_groups_launch(const MaskArgs &a, uint32_t lds, hipStream_
 
 template <int NK, int NP>
 static hipError_t mask_minex(const MaskArgs &a, uint32_t lds, hipStream_t s) {
-    if (a.md.minex == 2) return mask_groups_launch<NK, 2, NP>(a, lds, s);
-    if (a.md.minex == 3) return mask_groups_launch<NK, 3, NP>(a, lds, s);
-    if constexpr (NK <= 8) return mask_groups_launch<NK, 1, NP>(a, lds, s);  // wider tables: minex 2, 3 only
-    return hipErrorInvalidValue;
+    if constexpr (NP > kSeedMaxParts) {  // 10 or 12 rows: per-word counts only
+        if (a.md.minex == 3) return mask_groups_launch<NK, 3, NP>(a, lds, s);
+        return hipErrorInvalidValue;
+    } else {
+        if (a.md.minex == 2) return mask_groups_launch<NK, 2, NP>(a, lds, s);
+        if (a.md.minex == 3) return mask_groups_launch<NK, 3, NP>(a, lds, s);
+        if constexpr (NK <= 8) return mask_groups_launch<NK, 1, NP>(a, lds, s);  // wider tables: minex 2, 3 only
+        return hipErrorInvalidValue;
+    }
 }
 
 template <int NK>
 static hipError_t mask_parts(const MaskArgs &a, uint32_t lds, hipStream_t s) {
     if (a.md.parts <= 4) return mask_minex<NK, 4>(a, lds, s);
     if (a.md.parts <= 6) return mask_minex<NK, 6>(a, lds, s);
-    return mask_minex<NK, 8>(a, lds, s);
+    if (a.md.parts <= 8) return mask_minex<NK, 8>(a, lds, s);
+    if constexpr (NK >= kMaskWideWords) {  // wide rows only (narrower ones would spill registers)
+        if (a.md.parts <= 10) return mask_minex<NK, 10>(a, lds, s);
+        return mask_minex<NK, 12>(a, lds, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc &md, uint32_t cbase,
@@ -256,11 +279,14 @@ hipError_t launch_mask(const DevGenome &g, const PatternDesc &pd, const MaskDesc
                        uint32_t max_tiles) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
     if (limit == 0 || md.nrec == 0) return hipSuccess;
-    if (tile_stride == 0 || md.parts == 0 || md.parts > (uint32_t)kSeedMaxParts || md.minex < 1 || md.minex > 3 ||
+    // more than kSeedMaxParts ranges: per-word counts of wide tables only
+    const uint32_t max_parts =
+        md.minex == 3 && md.nk >= (uint32_t)kMaskWideWords ? (uint32_t)kMaskMaxParts : (uint32_t)kSeedMaxParts;
+    if (tile_stride == 0 || md.parts == 0 || md.parts > max_parts || md.minex < 1 || md.minex > 3 ||
         md.nrec > 32 * md.nk || md.nwords % 4 || md.recw % 4 || md.recw + 4 * md.nrec != md.nwords ||
         (uintptr_t)idx % 16)
         return hipErrorInvalidValue;
-    for (uint32_t p = 0; p < (uint32_t)kSeedMaxParts; p++) {
+    for (uint32_t p = 0; p < (uint32_t)kMaskMaxParts; p++) {
         const uint32_t span = (md.first[p] >> 8) & 0xFF, rbase = md.first[p] >> 16;
         if (p >= md.parts ? md.first[p] != md.zrow << 16  // padding: span 0 at the zero row
                           : span == 0 || span > (uint32_t)kMaskMaxSpan || (md.first[p] & 0xFF) + span > 32 ||

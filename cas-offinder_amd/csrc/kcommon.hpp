@@ -722,7 +722,7 @@ constexpr uint32_t kXor3 = 0x96;  // a ^ b ^ c
 // XOR3 and majority), then the comparison with need.
 template <int NP, int CH>
 __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w, uint32_t need) {
-    static_assert(NP == 4 || NP == 6 || NP == 8, "row counts 4, 6, 8");
+    static_assert(NP == 4 || NP == 6 || NP == 8 || NP == 10 || NP == 12, "row counts 4, 6, 8, 10, 12");
     auto fa = [](uint32_t x, uint32_t y, uint32_t z, uint32_t &sum, uint32_t &carry) {
         sum = __builtin_amdgcn_bitop3_b32(x, y, z, kXor3);
         carry = __builtin_amdgcn_bitop3_b32(x, y, z, kMaj);
@@ -740,7 +740,7 @@ __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w,
         if constexpr (NP == 6) {
             b0 = s1 ^ s2;
             fa(c1, c2, s1 & s2, b1, b2);
-        } else {
+        } else if constexpr (NP == 8) {
             const uint32_t s3 = m[6][w] ^ m[7][w], c3 = m[6][w] & m[7][w];
             uint32_t k1, t0, t1;
             fa(s1, s2, s3, b0, k1);
@@ -749,6 +749,27 @@ __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w,
             const uint32_t k2 = t0 & k1;
             b2 = t1 ^ k2;
             b3 = t1 & k2;
+        } else if constexpr (NP == 10) {
+            uint32_t s3, c3, x, k1, t0, t1, t2;
+            fa(m[6][w], m[7][w], m[8][w], s3, c3);
+            fa(s1, s2, s3, x, k1);  // weight 1: x, m9
+            b0 = x ^ m[9][w];
+            const uint32_t k2 = x & m[9][w];
+            fa(c1, c2, c3, t0, t1);  // weight 2: t0, k1, k2
+            fa(t0, k1, k2, b1, t2);
+            b2 = t1 ^ t2;  // weight 4: t1, t2 (at most 10 rows: no carry past weight 8)
+            b3 = t1 & t2;
+        } else {
+            uint32_t s3, c3, s4, c4, x, k1, y1, d1, y2, d2;
+            fa(m[6][w], m[7][w], m[8][w], s3, c3);
+            fa(m[9][w], m[10][w], m[11][w], s4, c4);
+            fa(s1, s2, s3, x, k1);  // weight 1: x, s4
+            b0 = x ^ s4;
+            const uint32_t k2 = x & s4;
+            fa(c1, c2, c3, y1, d1);  // weight 2: y1, c4, k1, k2
+            fa(c4, k1, k2, y2, d2);
+            b1 = y1 ^ y2;
+            fa(d1, d2, y1 & y2, b2, b3);  // weight 4 (at most 12 rows: b3 is weight 8)
         }
     }
     switch (need) {  // uniform: a scalar branch
@@ -760,6 +781,10 @@ __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w,
         case 6: return __builtin_amdgcn_bitop3_b32(b2, b1, b3, kAndOr);
         case 7: return __builtin_amdgcn_bitop3_b32(b2, b1 & b0, b3, kAndOr);
         case 8: return b3;
+        case 9: return b3 & __builtin_amdgcn_bitop3_b32(b0, b1, b2, kOr3);
+        case 10: return b3 & (b1 | b2);
+        case 11: return b3 & __builtin_amdgcn_bitop3_b32(b1, b0, b2, kAndOr);
+        case 12: return b3 & b2;
         default: return 0u;
     }
 }

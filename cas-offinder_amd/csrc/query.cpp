@@ -420,17 +420,17 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
     MaskDesc &d = s.desc;
     // ranges: consecutive groups of comparable positions, as equal as
     // possible, each spanning at most kMaskMaxSpan bits (holes included)
-    int parts = t.tmax + minex;
-    for (;; parts++) {
-        if (parts > kSeedMaxParts || parts > C) return false;
-        bool fits = true;
-        for (int p = 0; p < parts && fits; p++) {
+    auto fits = [&](int parts) {
+        if (parts > C) return false;
+        for (int p = 0; p < parts; p++) {
             const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
-            fits = g1 > g0 && pos[g1 - 1] + 1 - pos[g0] <= kMaskMaxSpan;
+            if (g1 <= g0 || pos[g1 - 1] + 1 - pos[g0] > kMaskMaxSpan) return false;
         }
-        if (fits) break;
-    }
-    d.parts = (uint32_t)parts;
+        return true;
+    };
+    int parts = t.tmax + minex;
+    while (parts <= kSeedMaxParts && !fits(parts)) parts++;
+    if (parts > kSeedMaxParts) return false;
     d.minex = (uint32_t)minex;
     d.nrec = (uint32_t)n;
     // records in key order, or, when minex 2 meets differing thresholds, by
@@ -448,7 +448,46 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
     d.care = care;
     d.nk = n <= 32 ? 1 : n <= 64 ? 2 : (uint32_t)(n + 127) / 128 * 4;  // 1, 2 or a multiple of 4
     if (minex == 1 && d.nk > 8) return false;
-    uint32_t rows = 0, rbase[kSeedMaxParts];
+    if (d.minex == 3 && d.nk >= (uint32_t)kMaskWideWords) {
+        // per-word counts of wide tables may use up to kMaskMaxParts ranges: more, shorter
+        // ranges raise each word's need (parts - its threshold) and cut the
+        // records that reach the exact count, at the price of more row reads
+        // and a wider count per word.  Pick the even part count (the kernel
+        // reads and counts an even number of rows) of least estimated cost
+        // per candidate: rows read (LDS issue of a 16-byte chunk ~8 cycles),
+        // the per-word counts, and the expected survivors of a random window
+        // (a range of c positions matches with probability 4^-c), each
+        // ~60 cycles through the divergent exact-count loop.
+        std::vector<int> tw(d.nk, -1);
+        for (size_t r = 0; r < n; r++) tw[r / 32] = std::max<int>(tw[r / 32], keys[order[r]].key->threshold);
+        auto cost = [&](int P) {
+            double pm[kMaskMaxParts + 1] = {1.0};  // distribution of exactly matched ranges
+            for (int p = 0; p < P; p++) {
+                const int c = (int)((int64_t)C * (p + 1) / P) - (int)((int64_t)C * p / P);
+                const double q = 1.0 / (double)(1u << (2 * c));
+                for (int k = p + 1; k >= 0; k--) pm[k] = pm[k] * (1 - q) + (k ? pm[k - 1] * q : 0.0);
+            }
+            double surv = 0;
+            for (size_t r = 0; r < n; r++) {
+                const int need = P - tw[r / 32];
+                for (int k = std::max(need, 0); k <= P; k++) surv += pm[k];
+            }
+            const int np = P <= 4 ? 4 : (P + 1) / 2 * 2;
+            const double count_ops = np <= 4 ? 6 : np <= 6 ? 11 : np <= 8 ? 15 : np <= 10 ? 18 : 22;
+            return d.nk * count_ops + 8.0 * np * ((d.nk + 3) / 4) + 60.0 * surv;
+        };
+        int best = parts;
+        const char *force = getenv("COF_MASK_PARTS");  // tests: a given part count where it fits
+        if (force && atoi(force) >= parts && atoi(force) <= kMaskMaxParts && fits(atoi(force))) {
+            best = atoi(force);
+        } else {
+            for (int P = parts + 1; P <= kMaskMaxParts; P++)
+                if (P % 2 == 0 && fits(P) && cost(P) < cost(best)) best = P;
+        }
+        parts = best;
+    }
+    d.parts = (uint32_t)parts;
+    uint32_t rows = 0, rbase[kMaskMaxParts];
     for (int p = 0; p < parts; p++) {
         const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
         const int a = pos[g0], span = pos[g1 - 1] + 1 - a;
@@ -457,7 +496,7 @@ bool build_mask_index(const std::vector<KeyRef> &keys, const KeyTable &t, int mi
         rows += 1u << (2 * span);
     }
     d.zrow = rows++;
-    for (int p = parts; p < kSeedMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
+    for (int p = parts; p < kMaskMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
     d.recw = (rows * d.nk + 3) & ~3u;
     d.nwords = d.recw + 4 * d.nrec;
     s.words.assign(d.nwords, 0);
@@ -845,7 +884,7 @@ bool build_bulge_index(const BulgeTable &t, MaskIndex &s) {
     std::vector<uint32_t> order(n);
     for (uint32_t r = 0; r < n; r++) order[r] = r;
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return tg[x] > tg[y]; });
-    uint32_t rows = 0, rbase[kSeedMaxParts];
+    uint32_t rows = 0, rbase[kMaskMaxParts];
     for (int p = 0; p < parts; p++) {
         const int g0 = (int)((int64_t)C * p / parts), g1 = (int)((int64_t)C * (p + 1) / parts);
         const int a = pos[g0], span = pos[g1 - 1] + 1 - a;
@@ -854,7 +893,7 @@ bool build_bulge_index(const BulgeTable &t, MaskIndex &s) {
         rows += 1u << (2 * span);
     }
     d.zrow = rows++;
-    for (int p = parts; p < kSeedMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
+    for (int p = parts; p < kMaskMaxParts; p++) d.first[p] = d.zrow << 16;  // span 0: the zero row
     d.recw = (rows * d.nk + 3) & ~3u;
     // rows, then per slot [glo, ghi, variant block offset, filter bias per alignment]
     d.nwords = d.recw + kBulgeSlotWords * n;

@@ -359,3 +359,64 @@ def test_seed_index_reversed_pam_holes_mixed_thresholds(ctx, tmp_path, n_guides,
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "reversed holes %d/%d" % (n_guides, holes))
     assert len(exp) > 0
+
+
+@pytest.mark.parametrize("parts", ["10", "12"])
+@pytest.mark.parametrize("n_guides,tmax,reverse,holes", [(400, 6, False, 0), (700, 3, False, 0), (500, 5, True, 2)])
+def test_mask_index_ten_and_twelve_ranges(tmp_path, parts, n_guides, tmax, reverse, holes):
+    """Wide tables (12-24 words of records) with thresholds 0..tmax mixed keep
+    per-word counts; COF_MASK_PARTS forces 10 or 12 ranges of 1-2 positions,
+    so a word needs up to 12 exact ranges (4-bit counts of 10 and 12 rows):
+    plain and batched records (variant 4 forced) equal the oracle's, with
+    planted near-sites at every threshold, on both PAM sides and with holes
+    (N inside ranges, listed under every code)."""
+    r = random.Random(1100 + n_guides + tmax + int(parts))
+    recs = [("c%d" % i, H.random_seq(r, r.randint(40000, 80000), "ACGTN" if i == 1 else "ACGT", n_runs=2))
+            for i in range(3)]
+    gp = str(tmp_path / "m.fa")
+    H.write_fasta(gp, recs)
+    pattern = "TTTV" + "N" * 23 if reverse else "N" * 20 + "NGG"
+    glen = 23 if reverse else 20
+    hole_pos = r.sample(range(3, glen - 2), holes)
+    guides = []
+    while len(guides) < n_guides // 2:  # copies of PAM sites with up to tmax substitutions
+        _, sq = r.choice(recs)
+        x = r.randrange(0, len(sq) - len(pattern))
+        w = sq[x:x + len(pattern)]
+        if not H.window_passes(pattern, w):
+            continue
+        core = list(w[4:] if reverse else w[:20])
+        if "N" in core:
+            continue
+        t = r.randint(0, tmax)
+        for _ in range(r.randint(0, t)):
+            core[r.randrange(glen)] = r.choice("ACGT")
+        guides.append((core, t))
+    while len(guides) < n_guides:
+        guides.append(([r.choice("ACGT") for _ in range(glen)], r.randint(0, tmax)))
+    r.shuffle(guides)
+    lines = [pattern]
+    for core, t in guides:
+        for p in hole_pos:
+            core[p] = "N"
+        lines.append(("NNNN%s %d" if reverse else "%sNNN %d") % ("".join(core), t))
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    saved = {k: os.environ.pop(k, None) for k in ("COF_FORCE_VARIANT", "COF_MASK_PARTS")}
+    os.environ["COF_FORCE_VARIANT"] = "4"
+    os.environ["COF_MASK_PARTS"] = parts
+    try:
+        c = casoff.Context(0)
+        try:
+            exp = check(c, tmp_path, gp, text, "parts %s, %d keys, tmax %d" % (parts, n_guides, tmax))
+            q = c.query(text)
+            c.search([q], casoff.MODE_BATCHED)
+            kernels = [rec["kernel"] for rec in c.launch_info()]
+        finally:
+            c.close()
+    finally:
+        for k, v in saved.items():
+            os.environ.pop(k, None)
+            if v is not None:
+                os.environ[k] = v
+    assert len(exp) > 0
+    assert any(", 3, %s, " % parts in k for k in kernels), kernels  # the 10- or 12-row kernel ran
