@@ -114,7 +114,10 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
 #pragma unroll
                         for (int w = 0; w < 4; w++) {
                             const uint32_t cw = 4 * c + w;
-                            sel[cw] = at_least<NP, 4>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
+                            // the branching comparison: its branches keep one chunk
+                            // of rows live at a time (without them the reads of every
+                            // chunk move ahead of the counts and spill)
+                            sel[cw] = at_least<NP, 4, false>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
                         }
                     } else {
                         uint32_t any[4], two[4];

@@ -720,7 +720,7 @@ constexpr uint32_t kXor3 = 0x96;  // a ^ b ^ c
 // Bits of word w set in at least `need` (uniform) of the NP rows: the
 // bit-sliced count b3 b2 b1 b0 of the set rows from full adders (v_bitop3
 // XOR3 and majority), then the comparison with need.
-template <int NP, int CH>
+template <int NP, int CH, bool BRANCHLESS = true>
 __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w, uint32_t need) {
     static_assert(NP == 4 || NP == 6 || NP == 8 || NP == 10 || NP == 12, "row counts 4, 6, 8, 10, 12");
     auto fa = [](uint32_t x, uint32_t y, uint32_t z, uint32_t &sum, uint32_t &carry) {
@@ -772,21 +772,38 @@ __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w,
             fa(d1, d2, y1 & y2, b2, b3);  // weight 4 (at most 12 rows: b3 is weight 8)
         }
     }
-    switch (need) {  // uniform: a scalar branch
-        case 1: return b0 | __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
-        case 2: return __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
-        case 3: return __builtin_amdgcn_bitop3_b32(b1, b0, b2 | b3, kAndOr);
-        case 4: return b2 | b3;
-        case 5: return __builtin_amdgcn_bitop3_b32(b2, b1 | b0, b3, kAndOr);
-        case 6: return __builtin_amdgcn_bitop3_b32(b2, b1, b3, kAndOr);
-        case 7: return __builtin_amdgcn_bitop3_b32(b2, b1 & b0, b3, kAndOr);
-        case 8: return b3;
-        case 9: return b3 & __builtin_amdgcn_bitop3_b32(b0, b1, b2, kOr3);
-        case 10: return b3 & (b1 | b2);
-        case 11: return b3 & __builtin_amdgcn_bitop3_b32(b1, b0, b2, kAndOr);
-        case 12: return b3 & b2;
-        default: return 0u;
+    if constexpr (!BRANCHLESS) {
+        switch (need) {  // uniform: a scalar branch per word
+            case 1: return b0 | __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
+            case 2: return __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
+            case 3: return __builtin_amdgcn_bitop3_b32(b1, b0, b2 | b3, kAndOr);
+            case 4: return b2 | b3;
+            case 5: return __builtin_amdgcn_bitop3_b32(b2, b1 | b0, b3, kAndOr);
+            case 6: return __builtin_amdgcn_bitop3_b32(b2, b1, b3, kAndOr);
+            case 7: return __builtin_amdgcn_bitop3_b32(b2, b1 & b0, b3, kAndOr);
+            case 8: return b3;
+            case 9: return b3 & __builtin_amdgcn_bitop3_b32(b0, b1, b2, kOr3);
+            case 10: return b3 & (b1 | b2);
+            case 11: return b3 & __builtin_amdgcn_bitop3_b32(b1, b0, b2, kAndOr);
+            case 12: return b3 & b2;
+            default: return 0u;
+        }
     }
+    // count >= need, bit-sliced from the low bit up, without branches: with
+    // n_i = bit i of the uniform need as an all-ones / all-zero scalar,
+    // ge_0 = b0 | ~n0 and ge_i = (b_i & ~n_i) | (~(b_i ^ n_i) & ge_{i-1}),
+    // one v_bitop3 each (truth tables over b_i * 4 + n_i * 2 + ge_{i-1}).
+    // A need above the largest count (15: a word without records) keeps nothing.
+    // The empty asm keeps the masks from being hoisted out of the candidate
+    // loop (four per word of a wide table held live would spill): they cost
+    // one scalar bit-field extract each where they are used.
+    asm volatile("" : "+s"(need));
+    auto nbit = [&](int i) { return (uint32_t)((int32_t)(need << (31 - i)) >> 31); };
+    constexpr uint32_t kGe0 = 0xF3, kGe = 0xB2;
+    uint32_t ge = __builtin_amdgcn_bitop3_b32(b0, nbit(0), 0u, kGe0);
+    ge = __builtin_amdgcn_bitop3_b32(b1, nbit(1), ge, kGe);
+    ge = __builtin_amdgcn_bitop3_b32(b2, nbit(2), ge, kGe);
+    return __builtin_amdgcn_bitop3_b32(b3, nbit(3), ge, kGe);
 }
 
 // One-word window planes read from the staged tile (LDS word index and
