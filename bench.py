@@ -278,6 +278,11 @@ def serve_bench(ctx, n_requests, qps, seed=3, verify=True):
            "wall_s": round(wall, 3), "guide_x_gbp_per_s": round(n_guides * ctx.info()[0] / 1e9 / wall, 2),
            "batches": st["batches"], "lane_hits": st["lane_hits"], "lane_grows": st["lane_grows"],
            "admission_waits": st["admission_waits"]}
+    if st["batches"]:
+        b = st["batches"]
+        out["per_batch"] = {"requests": round(st["requests"] / b, 2), "launches": round(st["launches"] / b, 2),
+                            "search_ms": round(st["t_search_ms"] / b, 3), "plan_ms": round(st["t_plan_ms"] / b, 3),
+                            "kernel_ms": round(st["t_kernel_ms"] / b, 3), "split_ms": round(st["t_split_ms"] / b, 3)}
     if verify:
         same_alone, same_plain, checked_plain = 0, 0, 0
         for (res, _), qs, (_, _, n) in zip(done, queries, reqs):
@@ -340,7 +345,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=20.0)
-    ap.add_argument("--serve-sweep", default="20,40,80,160,320",
+    ap.add_argument("--serve-sweep", default="20,40,80,120,160,240,320",
                     help="offered request rates of the served-capacity sweep ('' to skip)")
     ap.add_argument("--serve-p99-ms", type=float, default=100.0, help="p99 latency bound of the served capacity")
     ap.add_argument("--serve-plain", type=int, default=12, help="requests the plain path serves beside the sweep")

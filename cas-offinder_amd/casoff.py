@@ -57,6 +57,7 @@ class Stats(C.Structure):
         ("t_copy_ms", C.c_double),
         ("t_decode_ms", C.c_double),
         ("t_upload_ms", C.c_double),
+        ("t_plan_ms", C.c_double),
     ]
 
     def as_dict(self):
@@ -71,7 +72,8 @@ class ServerStats(C.Structure):
     _fields_ = [("lane_hits", C.c_uint64), ("batches", C.c_uint64), ("requests", C.c_uint64),
                 ("queries", C.c_uint64), ("hits", C.c_uint64), ("lane_grows", C.c_uint64),
                 ("admission_waits", C.c_uint64), ("peak_inflight_bytes", C.c_uint64),
-                ("hits_per_key", C.c_double)]
+                ("hits_per_key", C.c_double), ("launches", C.c_uint64), ("t_search_ms", C.c_double),
+                ("t_plan_ms", C.c_double), ("t_kernel_ms", C.c_double), ("t_split_ms", C.c_double)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -448,12 +448,15 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                    const ExtBufs *ext = nullptr, cof_result **pres = nullptr) {
     // the plan of the lane's previous search is reused for the same query
     // objects (it holds them, so their addresses identify them)
+    L.stats.t_plan_ms = 0;
     if (!(L.plan && L.plan->qs == qs && L.plan->gbase == gbase)) {
+        const auto tp0 = std::chrono::steady_clock::now();
         auto P = std::make_shared<BatchPlan>();
         if (const int rc = build_plan(qs, gbase, *P, err)) return rc;
         P->qs = qs;
         P->gbase = gbase;
         L.plan = std::move(P);
+        L.stats.t_plan_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - tp0).count();
     }
     const BatchPlan &P = *L.plan;
     const auto &ktab = P.ktab;

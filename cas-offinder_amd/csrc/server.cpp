@@ -118,7 +118,9 @@ void cof_server::worker(Lane *L) {
         cof_result *res = nullptr;
         std::string err;
         const uint64_t cap_before = L->d_hkey.n;
+        const auto ts0 = clk::now();
         const int rc = run_search(*L, ctx->dg, ctx->slab_positions, qs, COF_MODE_BATCHED, &ctx->tuner, &res, err);
+        const auto ts1 = clk::now();
         const bool grew = L->d_hkey.n > cap_before;
         const uint64_t found = rc == COF_OK ? res->hits.size() : 0;
         // split the merged result per request, query indices rebased to 0
@@ -150,6 +152,7 @@ void cof_server::worker(Lane *L) {
         }
         delete res;
         const auto now = clk::now();
+        const cof_stats bst = L->stats;
         {
             std::lock_guard<std::mutex> lk(mu);
             for (size_t i = 0; i < batch.size(); i++) {
@@ -165,6 +168,11 @@ void cof_server::worker(Lane *L) {
             st.queries += nq;
             st.hits += found;
             st.lane_grows += grew ? 1 : 0;
+            st.launches += bst.n_launches;
+            st.t_search_ms += std::chrono::duration<double, std::milli>(ts1 - ts0).count();
+            st.t_plan_ms += bst.t_plan_ms;
+            st.t_kernel_ms += bst.t_fused_ms;
+            st.t_split_ms += std::chrono::duration<double, std::milli>(now - ts1).count();
             if (rc == COF_OK && keys)  // running average of hits per key, weighted by keys
                 hits_per_key = std::max(1.0, 0.7 * hits_per_key + 0.3 * ((double)found / (double)keys));
             st.hits_per_key = hits_per_key;

@@ -76,6 +76,8 @@ typedef struct {
     double t_decode_ms;     /* collection: host decoding into cof_hit records (overlaps the copies) */
     double t_upload_ms;     /* the context's last genome upload: the three bit planes through pinned
                                staging buffers, plus the poison and tile tables (not per search) */
+    double t_plan_ms;       /* batched mode: host time building the search's key / bulge tables and
+                               seed indexes (0 when the lane reused the plan of the same queries) */
 } cof_stats;
 
 /* Synthetic genome specification (seeded, deterministic; no reference
@@ -254,8 +256,11 @@ int cof_server_stop(cof_server *s);
  * mem_budget_bytes at 20 device bytes per hit, 1 M to 64 M hits), served
  * batches / requests / queries / hits, batches whose hits outgrew the lane
  * (grown and rerun), admissions that waited for running batches, the peak
- * estimate of in-flight bytes, and the running average of hits per compare
- * key that sizes the admission estimate. */
+ * estimate of in-flight bytes, the running average of hits per compare
+ * key that sizes the admission estimate, and, summed over the batches, their
+ * launches and the milliseconds of the batched search (wall time), of its
+ * host table building, of its kernels (HIP events) and of splitting the
+ * merged records per request. */
 typedef struct {
     uint64_t lane_hits;
     uint64_t batches, requests, queries, hits;
@@ -263,6 +268,8 @@ typedef struct {
     uint64_t admission_waits;
     uint64_t peak_inflight_bytes;
     double hits_per_key;
+    uint64_t launches;
+    double t_search_ms, t_plan_ms, t_kernel_ms, t_split_ms;
 } cof_server_stats;
 int cof_server_stats_get(cof_server *s, cof_server_stats *out);
 
