@@ -2,6 +2,7 @@
 #include "engine.hpp"
 
 #include <algorithm>
+#include <cmath>
 #include <chrono>
 #include <cstring>
 #include <functional>
@@ -136,6 +137,27 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     return COF_OK;
 }
 
+// Probability that a uniform random base matches pattern code c (IUPAC:
+// README.md:139-151).
+static double base_match_prob(char c) {
+    switch (c) {
+        case 'A': case 'C': case 'G': case 'T': return 0.25;
+        case 'R': case 'Y': case 'K': case 'M': case 'S': case 'W': return 0.5;
+        case 'B': case 'D': case 'H': case 'V': return 0.75;
+        default: return 1.0;
+    }
+}
+
+// P(at most t of L bases mismatch), each independently with probability 3/4.
+static double at_most_mismatches(int L, int t) {
+    double term = std::pow(0.25, L), sum = 0;  // k = 0
+    for (int k = 0; k <= std::min(t, L); k++) {
+        sum += term;
+        term *= (double)(L - k) / (k + 1) * 3.0;  // C(L, k+1) / C(L, k) x (3/4) / (1/4)
+    }
+    return std::min(sum, 1.0);
+}
+
 // Tables, indexes and launch signatures of a batched search over qs (see
 // search_batched).
 static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::vector<uint32_t> &gbase,
@@ -160,15 +182,40 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
         if (qs[qi]->keys.empty()) continue;
         (qs[qi]->bulge.empty() ? plain : bulged).push_back(qi);
     }
+    // bulge groups: the queries of one pattern (without its bulge padding)
+    // and PAM side share a launch whatever their bulge sizes; where the
+    // union of their alignments exceeds kMaxShifts or that table does not
+    // qualify, one launch per (D, R)
     std::vector<BulgeTable> &btab = P.btab;
-    for (auto &grp : group_by(bulged, [](const Query &q) {
-             return q.pattern + "|" + std::to_string(q.D) + "|" + std::to_string(q.R) + (q.rev ? "|r" : "|f");
-         })) {
-        std::vector<BulgeRef> refs;
-        for (size_t qi : grp) refs.push_back({qs[qi].get(), gbase[qi]});
-        BulgeTable t;
-        if (build_bulge_table(refs, t) == 1) btab.push_back(std::move(t));
-        else plain.insert(plain.end(), grp.begin(), grp.end());
+    std::vector<std::vector<size_t>> bq;  // queries of each bulge table
+    auto dr_key = [](const Query &q) {
+        return q.pattern + "|" + std::to_string(q.D) + "|" + std::to_string(q.R) + (q.rev ? "|r" : "|f");
+    };
+    for (auto &grp : group_by(bulged, [](const Query &q) { return unpadded_pattern(q) + (q.rev ? "|r" : "|f"); })) {
+        uint32_t dmax = 0, rmax = 0;
+        for (size_t qi : grp) dmax = std::max(dmax, qs[qi]->D), rmax = std::max(rmax, qs[qi]->R);
+        const auto parts = group_by(grp, dr_key);
+        if (parts.size() > 1 && dmax + rmax + 1 <= (uint32_t)kMaxShifts) {
+            std::vector<BulgeRef> refs;
+            for (size_t qi : grp) refs.push_back({qs[qi].get(), gbase[qi]});
+            BulgeTable t;
+            if (build_bulge_table(refs, t) == 1) {
+                btab.push_back(std::move(t));
+                bq.push_back(grp);
+                continue;
+            }
+        }
+        for (auto &part : parts) {
+            std::vector<BulgeRef> refs;
+            for (size_t qi : part) refs.push_back({qs[qi].get(), gbase[qi]});
+            BulgeTable t;
+            if (build_bulge_table(refs, t) == 1) {
+                btab.push_back(std::move(t));
+                bq.push_back(part);
+            } else {
+                plain.insert(plain.end(), part.begin(), part.end());
+            }
+        }
     }
     std::sort(plain.begin(), plain.end());
     const auto kgroups = group_by(plain, [](const Query &q) { return q.pattern; });
@@ -275,6 +322,31 @@ static int build_plan(const std::vector<std::shared_ptr<Query>> &qs, const std::
                      bucket(t.nguides) + "|" + std::to_string(t.tmax);
         }
     }
+    // launch order: most expected hits first.  Each launch's hits are
+    // sorted, copied and decoded while the later launches run, so the last
+    // launch's collection is what remains after the kernels: it should be
+    // the one with the fewest hits.  Expected hits on a uniform random
+    // genome: PAM match probability x the sum over compare keys of
+    // P(at most t of the key's L bases mismatch), each with probability 3/4.
+    std::vector<double> est(nlaunch, 0.0);
+    auto add_query = [&](size_t i, const Query &q) {
+        double pam = 1.0;
+        for (char c : q.pattern) pam *= base_match_prob(c);
+        double s = 0;
+        for (const CompareKey &k : q.keys) {
+            int L = 0;
+            for (char c : k.seq) L += c == 'A' || c == 'C' || c == 'G' || c == 'T';
+            s += at_most_mismatches(L, k.threshold);
+        }
+        est[i] += pam * s;
+    };
+    for (size_t g = 0; g < kgroups.size(); g++)
+        for (size_t qi : kgroups[g]) add_query(g, *qs[qi]);
+    for (size_t g = 0; g < bq.size(); g++)
+        for (size_t qi : bq[g]) add_query(ktab.size() + g, *qs[qi]);
+    P.order.resize(nlaunch);
+    for (size_t i = 0; i < nlaunch; i++) P.order[i] = (uint32_t)i;
+    std::stable_sort(P.order.begin(), P.order.end(), [&](uint32_t x, uint32_t y) { return est[x] > est[y]; });
     return COF_OK;
 }
 
@@ -476,6 +548,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     const auto &bslice = P.bslice;
     const auto &has_bslice = P.has_bslice;
     const auto &bsoff = P.bsoff;
+    const auto &ord = P.order;
     const size_t nlaunch = ktab.size() + btab.size();
     // device buffers: the lane's, or the caller's (no allocation, no graphs,
     // no trials, no per-launch timing)
@@ -527,11 +600,13 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(t.cb, sizeof t.cb);
             put(t.gap, sizeof t.gap);
             put(&t.pat, sizeof(PatternDesc));
+            put(&t.dpad, sizeof t.dpad);
             put(&boff[g], sizeof boff[g]);
             put(&bioff[g], sizeof bioff[g]);
             put(&bsoff[g], sizeof bsoff[g]);
         }
         put(kw.data(), kw.size() * 4);
+        put(ord.data(), ord.size() * sizeof ord[0]);
     }
     const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key;
     if (!replay && !kw.empty())
@@ -539,38 +614,42 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
     if (kCandBase + nlaunch * kCandWords > count_n) { err = "internal: counter buffer too small"; return COF_E_INVALID; }
     auto cand_slots = [&](size_t i) { return d_count + kCandBase + i * kCandWords; };
-    // launch i < ktab.size(): fused key group i; else bulge group i - ktab.size()
-    auto launch = [&](size_t i, int v, uint32_t stride, uint32_t max_tiles) {
+    // launch slot j runs plan launch i = ord[j]: i < ktab.size() is fused key
+    // group i, else bulge group i - ktab.size(); counters, events, graphs and
+    // variants are per slot
+    auto launch = [&](size_t j, int v, uint32_t stride, uint32_t max_tiles) {
+        const size_t i = ord[j];
         if (i < ktab.size() && (v == 3 || v == 4) && has_mask[v - 3][i])
             return launch_mask(dg, krep[i]->pat, masks[v - 3][i].desc, (uint32_t)ktab[i].cbase,
-                               d_keys + moff[v - 3][i], d_hkey, d_hmm, d_count, cand_slots(i),
+                               d_keys + moff[v - 3][i], d_hkey, d_hmm, d_count, cand_slots(j),
                                cap, L.stream, stride, max_tiles);
         if (i < ktab.size() && v == 2 && has_seed[i])
             return launch_seed(dg, krep[i]->pat, seeds[i].desc, seeds[i].lds_bytes, (uint32_t)ktab[i].cbase,
                                d_keys + koff[i], d_keys + soff[i], d_hkey, d_hmm, d_count,
-                               cand_slots(i), cap, L.stream, stride, max_tiles);
+                               cand_slots(j), cap, L.stream, stride, max_tiles);
         if (i < ktab.size())
             return launch_fused(dg, krep[i]->pat, ktab[i].nw, ktab[i].gen, (uint32_t)ktab[i].cbase,
                                 d_keys + koff[i], ktab[i].units, 0, d_hkey, d_hmm, d_count,
-                                cand_slots(i), cap, L.stream, v, stride, max_tiles);
+                                cand_slots(j), cap, L.stream, v, stride, max_tiles);
         const size_t bg = i - ktab.size();
         const BulgeTable &t = btab[bg];
         if (v == 3 && has_bslice[bg])
-            return launch_bulge_slice(dg, t.pat, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg], bslice[bg].desc,
-                                      d_keys + bsoff[bg], d_hkey, d_hmm, d_count, cand_slots(i),
+            return launch_bulge_slice(dg, t.pat, t.ns, t.cb, t.gap, t.rev, t.dpad, d_keys + boff[bg], bslice[bg].desc,
+                                      d_keys + bsoff[bg], d_hkey, d_hmm, d_count, cand_slots(j),
                                       cap, L.stream, stride, max_tiles);
         if (v == 2 && has_bidx[bg])
-            return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg], bidx[bg].desc,
-                                      d_keys + bioff[bg], d_hkey, d_hmm, d_count, cand_slots(i),
+            return launch_bulge_index(dg, t.pat, t.ns, t.cb, t.gap, t.rev, t.dpad, d_keys + boff[bg], bidx[bg].desc,
+                                      d_keys + bioff[bg], d_hkey, d_hmm, d_count, cand_slots(j),
                                       cap, L.stream, stride, max_tiles);
-        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, d_keys + boff[bg],
-                            t.units, d_hkey, d_hmm, d_count, cand_slots(i), cap, L.stream, v,
+        return launch_bulge(dg, t.pat, t.nw, t.gen, t.ns, t.cb, t.gap, t.rev, t.dpad, d_keys + boff[bg],
+                            t.units, d_hkey, d_hmm, d_count, cand_slots(j), cap, L.stream, v,
                             stride, max_tiles);
     };
     std::vector<int> variant(nlaunch, 0);
     // variants launch i can run (a cached choice for the same signature may
     // name an index the table does not qualify for: then the default runs)
-    auto avail = [&](size_t i, int v) {
+    auto avail = [&](size_t j, int v) {
+        const size_t i = ord[j];
         if (i >= ktab.size())
             return v == 0 || v == 1 || (v == 2 && has_bidx[i - ktab.size()]) || (v == 3 && has_bslice[i - ktab.size()]);
         switch (v) {
@@ -585,10 +664,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         variant = L.graph_variants;
     } else if (tuner && tuner->force >= 0) {
         for (size_t i = 0; i < nlaunch; i++)
-            variant[i] = avail(i, tuner->force) ? tuner->force : i < ktab.size() ? 0 : 1;
+            variant[i] = avail(i, tuner->force) ? tuner->force : ord[i] < ktab.size() ? 0 : 1;
     } else if (tuner && !(tuner->enabled && L.trials && !ext)) {
         for (size_t i = 0; i < nlaunch; i++)  // the nearest known decision, else the default
-            if (!tuner->get_near(sig[i], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
+            if (!tuner->get_near(sig[ord[i]], variant[i]) || !avail(i, variant[i])) variant[i] = 0;
     } else if (tuner) {
         // time every variant of each new signature on tiles spread over the
         // genome; keep the default unless another is more than 3% faster
@@ -598,8 +677,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         bool tried = false;
         const auto tt0 = std::chrono::steady_clock::now();
         for (size_t i = 0; i < nlaunch; i++) {
-            if (tuner->get(sig[i], variant[i]) && avail(i, variant[i])) continue;
-            const int nv = i < ktab.size() ? kFusedVariants : kBulgeVariants;
+            if (tuner->get(sig[ord[i]], variant[i]) && avail(i, variant[i])) continue;
+            const int nv = ord[i] < ktab.size() ? kFusedVariants : kBulgeVariants;
             double best_ms = 0;
             int best = 0;
             for (int v = 0; v < nv; v++) {
@@ -616,7 +695,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
                 else if (ms < 0.97 * best_ms) best_ms = ms, best = v;
             }
             variant[i] = best;
-            tuner->set(sig[i], best);
+            tuner->set(sig[ord[i]], best);
             tried = true;
         }
         if (tried)  // trial hits and candidate counts are discarded
@@ -626,7 +705,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     if (ext) {
         // eager launches and the hit counter back: no events, graphs or allocation
         for (size_t i = 0; i < nlaunch; i++)
-            HIP_TRY(launch(i, variant[i], 1, 0), i < ktab.size() ? "fused launch" : "bulge launch");
+            HIP_TRY(launch(i, variant[i], 1, 0), ord[i] < ktab.size() ? "fused launch" : "bulge launch");
         unsigned long long h = 0;
         HIP_TRY(hipMemcpyAsync(&h, d_count, sizeof h, hipMemcpyDeviceToHost, L.stream), "hipMemcpyAsync(count)");
         HIP_TRY(hipStreamSynchronize(L.stream), "fused");
@@ -734,7 +813,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         }
     } else {
         for (size_t i = 0; i < nlaunch; i++)
-            HIP_TRY(enqueue(i), i < ktab.size() ? "fused launch" : "bulge launch");
+            HIP_TRY(enqueue(i), ord[i] < ktab.size() ? "fused launch" : "bulge launch");
     }
     HIP_TRY(hipEventRecord(L.ev[1], L.stream), "hipEventRecord");
     std::vector<unsigned long long> raw(kCandBase + nlaunch * kCandWords);
@@ -754,7 +833,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     for (size_t i = 0; i < nlaunch; i++) {
         LaunchRec &lr = L.launches[i];
         lr.kernel = kernels[i];
-        lr.sig = sig[i];
+        lr.sig = sig[ord[i]];
         lr.variant = variant[i];
         lr.ms = ms_between(L.lev[2 * i], L.lev[2 * i + 1]);
         lr.candidates = cnt[2 + i];
@@ -765,8 +844,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         L.stats.t_comparer_ms = launch_ms;
     }
     uint64_t ncand = 0, npairs = 0, nbase = 0, nmodel = 0;
+    std::vector<size_t> slot(nlaunch);  // slot of each plan launch
+    for (size_t j = 0; j < nlaunch; j++) slot[ord[j]] = j;
     for (size_t g = 0; g < ktab.size(); g++) {
-        const uint64_t c = cnt[2 + g];
+        const uint64_t c = cnt[2 + slot[g]];
         ncand += c;
         npairs += c * ktab[g].nkeys;
         nbase += c * ktab[g].care_sum;
@@ -774,7 +855,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     }
     for (size_t g = 0; g < btab.size(); g++) {
         const BulgeTable &t = btab[g];
-        const uint64_t c = cnt[2 + ktab.size() + g];
+        const uint64_t c = cnt[2 + slot[ktab.size() + g]];
         ncand += c;
         npairs += c * t.nkeys;
         nbase += c * t.care_sum;

@@ -110,16 +110,17 @@ hipError_t launch_fused(const DevGenome &g, const PatternDesc &pd, int nw, bool 
                         unsigned long long *count, unsigned long long *cand_count, uint64_t cap, hipStream_t s,
                         int variant, uint32_t tile_stride, uint32_t max_tiles);
 hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
-                        const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
+                        const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, uint32_t nunits,
+                        uint64_t *hkey,
                         uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
                         hipStream_t s, int variant, uint32_t tile_stride, uint32_t max_tiles);
 hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
-                              const uint32_t *gap, bool rev, const uint32_t *table, const MaskDesc &md,
+                              const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, const MaskDesc &md,
                               const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                               unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
                               uint32_t max_tiles);
 hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
-                              const uint32_t *gap, bool rev, const uint32_t *table, const SliceDesc &sd,
+                              const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, const SliceDesc &sd,
                               const uint32_t *slices, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                               unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
                               uint32_t max_tiles);
@@ -206,6 +207,7 @@ struct BatchPlan {
     std::vector<uint32_t> kw;
     std::vector<size_t> koff, soff, boff, moff[2], bioff;
     std::vector<std::string> sig;
+    std::vector<uint32_t> order;  // launch order: plan launch of each slot (most expected hits first)
 };
 
 // One in-order stream with its own work buffers; searches on different lanes

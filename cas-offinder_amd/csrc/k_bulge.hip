@@ -14,7 +14,7 @@ constexpr int kBulgeQueue = 64;
 
 // QUEUE = false runs the exact counts in place, for the lanes that passed
 // (fewer registers; cheaper when filters rarely pass).
-template <int NW, bool GEN, int NS, bool QUEUE>
+template <int NW, bool GEN, int NS, bool QUEUE, bool PAD>
 __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge_kernel(BulgeArgs a) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
@@ -26,6 +26,8 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
     const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave], *qm = s_qm[wave];
     const kptr_t tab = (kptr_t)a.table;
+    extern __shared__ __align__(16) uint32_t s_sep[];  // separator bitmap, kLdsWords words (PAD)
+    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     tile_candidates<1, true>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
                              stage_ref(s_stage), a.hkey, a.hmm, a.count, a.cap,
                              [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
@@ -44,11 +46,11 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
 #pragma unroll
                     for (int w = 0; w < NW; w++) M[u][w] = qm[lane * NS * NW + u * NW + w];
                 const kptr_t vb = tab + kp[KP];
-                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
-                                    stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
+                TileSink sink;
+                const bool ok = bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink);
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
-                    if ((fam >> u) & 1) {
+                    if (ok && ((fam >> u) & 1)) {
                         const bool lft0 = u == 0 || a.rev;
                         uint32_t left[NW], right[NW];
 #pragma unroll
@@ -123,11 +125,11 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
                     if (pass) {
                         const kptr_t vb = tab + kp[KP];
                         const uint32_t ce = e[0];
-                        const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
-                                            stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
+                        TileSink sink;
+                        const bool ok = bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink);
 #pragma unroll
                         for (int u = 0; u < NS; u++) {
-                            if (f[u] < 0) {
+                            if (ok && f[u] < 0) {
                                 const bool lft0 = u == 0 || a.rev;
                                 uint32_t left[NW], right[NW];
 #pragma unroll
@@ -172,9 +174,12 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
 
 template <int NW, bool GEN, int NS>
 static void bulge_variant(int variant, dim3 grid, const BulgeArgs &a, hipStream_t s) {
-    note_kernel("bulge_kernel", NW, GEN, NS, variant != 1);
-    if (variant == 1) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, false>), grid, dim3(kThreads), 0, s, a);
-    else hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, true>), grid, dim3(kThreads), 0, s, a);
+    note_kernel("bulge_kernel", NW, GEN, NS, variant != 1, a.pad != 0);
+    const uint32_t lds = a.pad ? kSepBytes : 0u;
+    if (variant == 1 && a.pad) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, false, true>), grid, dim3(kThreads), lds, s, a);
+    else if (variant == 1) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, false, false>), grid, dim3(kThreads), lds, s, a);
+    else if (a.pad) hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, true, true>), grid, dim3(kThreads), lds, s, a);
+    else hipLaunchKernelGGL((bulge_kernel<NW, GEN, NS, true, false>), grid, dim3(kThreads), lds, s, a);
 }
 
 template <int NW, bool GEN>
@@ -190,7 +195,8 @@ static hipError_t launch_bulge_ns(int ns, int variant, dim3 grid, const BulgeArg
 }
 
 hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool gen, int ns, const int32_t *cb,
-                        const uint32_t *gap, bool rev, const uint32_t *table, uint32_t nunits, uint64_t *hkey,
+                        const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, uint32_t nunits,
+                        uint64_t *hkey,
                         uint16_t *hmm, unsigned long long *count, unsigned long long *cand_count, uint64_t cap,
                         hipStream_t s, int variant, uint32_t tile_stride, uint32_t max_tiles) {
     const uint64_t limit = g.total >= pd.P ? g.total - pd.P + 1 : 0;
@@ -204,6 +210,7 @@ hipError_t launch_bulge(const DevGenome &g, const PatternDesc &pd, int nw, bool 
         a.gap[u] = gap[u];
     }
     a.rev = rev ? 1u : 0u;
+    a.pad = pad;
     a.nunits = nunits;
     a.tile_stride = tile_stride;
     a.table = table;

@@ -21,7 +21,7 @@ struct BulgeIndexArgs {
     const uint32_t *idx;
 };
 
-template <int NK, int NS, int NP>
+template <int NK, int NS, int NP, bool PAD>
 __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(BulgeIndexArgs ia) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
@@ -29,6 +29,8 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
     __shared__ HitStageBuf<kStageCap> s_stage;
     extern __shared__ __align__(16) uint32_t s_idx[];
     const BulgeArgs &a = ia.b;
+    uint32_t *s_sep = s_idx + ia.md.nwords;  // separator bitmap, kLdsWords words (PAD)
+    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     const MaskDesc &md = ia.md;
     for (uint32_t i = threadIdx.x; i < md.nwords / 4; i += kThreads)
         reinterpret_cast<uint4 *>(s_idx)[i] = reinterpret_cast<const uint4 *>(ia.idx)[i];
@@ -98,8 +100,8 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
                 }
                 if (acc >= 0) continue;
                 const kptr_t vb = tab + s0.z;
-                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
-                                    stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
+                TileSink sink;
+                if (!bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink)) continue;
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if (fl[u] < 0) {
@@ -115,10 +117,18 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
 
 template <int NK, int NS>
 static hipError_t bulge_index_parts(uint32_t parts, dim3 grid, const BulgeIndexArgs &a, uint32_t lds, hipStream_t s) {
-    note_kernel("bulge_index_kernel", NK, NS, parts <= 4 ? 4 : parts <= 6 ? 6 : 8);
-    if (parts <= 4) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 4>), grid, dim3(kThreads), lds, s, a);
-    else if (parts <= 6) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 6>), grid, dim3(kThreads), lds, s, a);
-    else hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 8>), grid, dim3(kThreads), lds, s, a);
+    const bool pad = a.b.pad != 0;
+    note_kernel("bulge_index_kernel", NK, NS, parts <= 4 ? 4 : parts <= 6 ? 6 : 8, pad);
+    if (parts <= 4) {
+        if (pad) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 4, true>), grid, dim3(kThreads), lds, s, a);
+        else hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 4, false>), grid, dim3(kThreads), lds, s, a);
+    } else if (parts <= 6) {
+        if (pad) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 6, true>), grid, dim3(kThreads), lds, s, a);
+        else hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 6, false>), grid, dim3(kThreads), lds, s, a);
+    } else {
+        if (pad) hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 8, true>), grid, dim3(kThreads), lds, s, a);
+        else hipLaunchKernelGGL((bulge_index_kernel<NK, NS, 8, false>), grid, dim3(kThreads), lds, s, a);
+    }
     return hipGetLastError();
 }
 
@@ -135,7 +145,7 @@ static hipError_t bulge_index_ns(int ns, uint32_t parts, dim3 grid, const BulgeI
 }
 
 hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
-                              const uint32_t *gap, bool rev, const uint32_t *table, const MaskDesc &md,
+                              const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, const MaskDesc &md,
                               const uint32_t *idx, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                               unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
                               uint32_t max_tiles) {
@@ -165,6 +175,7 @@ hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns,
         b.gap[u] = gap[u];
     }
     b.rev = rev ? 1u : 0u;
+    b.pad = pad;
     b.nunits = 1;
     b.tile_stride = tile_stride;
     b.table = table;
@@ -177,7 +188,7 @@ hipError_t launch_bulge_index(const DevGenome &g, const PatternDesc &pd, int ns,
     a.md = md;
     a.idx = idx;
     const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
-    const uint32_t lds = md.nwords * 4;
+    const uint32_t lds = md.nwords * 4 + (pad ? kSepBytes : 0u);
     switch (md.nk) {
         case 1: return bulge_index_ns<1>(ns, md.parts, grid, a, lds, s);
         case 2: return bulge_index_ns<2>(ns, md.parts, grid, a, lds, s);

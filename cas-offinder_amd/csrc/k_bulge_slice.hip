@@ -14,7 +14,10 @@
 // 650-682), so its mismatches are at least popc(M_0 & M_u) - gap; the family
 // cannot hit unless the count of the planes M_0[k] & M_u[k] is at most
 // threshold + gap, and the no-bulge key (popc(M_0) <= its threshold) is
-// covered by the first family's count, which can only be smaller.  The
+// covered by the count of the guide's first family, which can only be
+// smaller.  Slots are grouped by their family sets (a table may merge
+// queries of different bulge sizes) and a word skips a family none of its
+// guides has.  The
 // counts of 32 guides come from one carry-save adder network over the kn
 // planes (slicecount.hpp) and one bit-sliced comparison with the guides'
 // bounds.  For words of large bounds (slots are sorted by bound) each family
@@ -42,7 +45,7 @@ constexpr int kSliceQueue = 64;
 // DN / DP: window shifts -1..-DN and +1..+DP of the filter families (the
 // alignments of the table other than 0); KN: care positions of the guide
 // words (the table's kn is in (KN - 4, KN]).
-template <int DN, int DP, int KN>
+template <int DN, int DP, int KN, bool PAD>
 __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
     constexpr int NF = DN + DP, NS = NF + 1, OFF = DN > DP ? DN : DP, PN = KN + 2 * OFF;
     constexpr int FW = PN <= 32 ? 1 : 2;  // frame words of the broadcast window positions
@@ -54,6 +57,8 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
     extern __shared__ __align__(16) uint32_t s_slot[];  // nslots x kBulgeSlotWords
     const BulgeArgs &a = sa.b;
     const SliceDesc &sd = sa.sd;
+    uint32_t *s_sep = s_slot + sd.nslots * kBulgeSlotWords;  // separator bitmap, kLdsWords words (PAD)
+    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     for (uint32_t i = threadIdx.x; i < sd.nslots * kBulgeSlotWords / 4; i += kThreads)
         reinterpret_cast<uint4 *>(s_slot)[i] = reinterpret_cast<const uint4 *>(sa.slices + sd.slot_off)[i];
     // visible after tile_candidates' first barrier
@@ -66,7 +71,7 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
     uint32_t qn = 0;  // queued pairs of this wave (uniform); the tile stays staged across its passes
 
     // exact counts of the first n queued pairs, one per lane
-    auto drain = [&](uint64_t t0, auto rd, uint32_t n) {
+    auto drain = [&](uint64_t t0, auto rd, uint32_t n) __attribute__((always_inline)) {
         __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
         __builtin_amdgcn_wave_barrier();
         __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -91,10 +96,9 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
                 fl[u] = (int32_t)(bias[u] + (uint32_t)__popc(u == 0 ? M[0] : M[u] & M[0]));
                 acc |= fl[u];
             }
-            if (acc < 0) {
+            TileSink sink;
+            if (acc < 0 && bulge_sink<PAD>(a, t0, ce, tab + s0.z, stage_ref(s_stage), s_sep, sink)) {
                 const kptr_t vb = tab + s0.z;
-                const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ce >> 1), ce & 1, vb[0]},
-                                    stage_ref(s_stage), ((ce & 1) << 15) | (ce >> 1)};
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if (fl[u] < 0) {
@@ -141,6 +145,7 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
                 for (int f = 0; f < NF; f++) {
                     const int d = f < DN ? -(f + 1) : f - DN + 1;
                     const kptr_t tp = blk + 2 * KN + (kSliceTPlanes + 1) * f;
+                    if (tp[kSliceTPlanes] == 0) continue;  // no guide of the word has this family (uniform)
                     const uint32_t t[6] = {tp[0], tp[1], tp[2], tp[3], tp[4], tp[5]};
                     // guides' mismatches at alignment u, care position k
                     auto mu = [&](int k) {
@@ -222,8 +227,10 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
 
 template <int DN, int DP, int KN>
 static hipError_t slice_launch(dim3 grid, const SliceArgs &a, uint32_t lds, hipStream_t s) {
-    note_kernel("bulge_slice_kernel", DN, DP, KN);
-    hipLaunchKernelGGL((bulge_slice_kernel<DN, DP, KN>), grid, dim3(kThreads), lds, s, a);
+    const bool pad = a.b.pad != 0;
+    note_kernel("bulge_slice_kernel", DN, DP, KN, pad);
+    if (pad) hipLaunchKernelGGL((bulge_slice_kernel<DN, DP, KN, true>), grid, dim3(kThreads), lds, s, a);
+    else hipLaunchKernelGGL((bulge_slice_kernel<DN, DP, KN, false>), grid, dim3(kThreads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -240,7 +247,7 @@ static hipError_t slice_kn(dim3 grid, const SliceArgs &a, uint32_t lds, hipStrea
 }
 
 hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns, const int32_t *cb,
-                              const uint32_t *gap, bool rev, const uint32_t *table, const SliceDesc &sd,
+                              const uint32_t *gap, bool rev, uint32_t pad, const uint32_t *table, const SliceDesc &sd,
                               const uint32_t *slices, uint64_t *hkey, uint16_t *hmm, unsigned long long *count,
                               unsigned long long *cand_count, uint64_t cap, hipStream_t s, uint32_t tile_stride,
                               uint32_t max_tiles) {
@@ -259,6 +266,7 @@ hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns,
         b.gap[u] = gap[u];
     }
     b.rev = rev ? 1u : 0u;
+    b.pad = pad;
     b.nunits = 1;
     b.tile_stride = tile_stride;
     b.table = table;
@@ -271,7 +279,7 @@ hipError_t launch_bulge_slice(const DevGenome &g, const PatternDesc &pd, int ns,
     a.sd = sd;
     a.slices = slices;
     const dim3 grid = tile_grid(limit, tile_stride, max_tiles);
-    const uint32_t lds = sd.nslots * kBulgeSlotWords * 4;
+    const uint32_t lds = sd.nslots * kBulgeSlotWords * 4 + (pad ? kSepBytes : 0u);
     switch (sd.dn * 3 + sd.dp) {
         case 1: return slice_kn<0, 1>(grid, a, lds, s);
         case 2: return slice_kn<0, 2>(grid, a, lds, s);

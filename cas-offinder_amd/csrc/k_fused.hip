@@ -37,7 +37,7 @@ __global__ __launch_bounds__(kThreads) void fused_kernel(FusedArgs a) {
                                                                      32 * kHaloL;
                                window_planes<NW>(rd, bit, strand, l[w], h[w], o[w]);
                                sink[w] = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, a.gbase},
-                                                  stage_ref(s_stage), (strand << 15) | ls};
+                                                  stage_ref(s_stage), stage_low(strand, (int32_t)ls)};
                            }
                            score_table<NW, GEN, W, BLK>((kptr_t)a.table, a.nunits, l, h, o, sink, valid);
                        });
@@ -102,7 +102,7 @@ __global__ __launch_bounds__(kThreads) void fused_seed_kernel(SeedArgs a) {
             window_planes<1>(rd, bit, strand, l, h, o);
             const uint32_t lc = l[0] & care, hc = h[0] & care, oc = o[0] & care;
             const TileSink sink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + ls, strand, 0u}, stage_ref(s_stage),
-                                (strand << 15) | ls};
+                                stage_low(strand, (int32_t)ls)};
             for (uint32_t p = 0; p < sd.parts; p++) {
                 const uint32_t pm = sd.mask[p];
                 if (oc & pm) continue;  // a genome N at a comparable position: the range cannot match exactly

@@ -94,7 +94,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fu
                     uint32_t ev = e[0];
                     asm volatile("" : "+v"(ev));  // keeps the sink's arithmetic on the (rare) hit path
                     emit_hit(TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + (ev >> 1), ev & 1, 0u},
-                                      stage_ref(g_stage), ((ev & 1) << 15) | (ev >> 1)},
+                                      stage_ref(g_stage), stage_low(ev & 1, (int32_t)(ev >> 1))},
                              k.w, mm);
                 }
             };

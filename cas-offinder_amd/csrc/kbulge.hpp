@@ -25,6 +25,7 @@ struct BulgeArgs {
     int32_t cb[kMaxShifts];    // window position of mask bit 0 at each shift
     uint32_t gap[kMaxShifts];  // RNA-bulge length of the family of each shift
     uint32_t rev;              // reversed PAM: left part at shift 0, right part at shift u
+    uint32_t pad;              // largest extra padding of a guide (0: every guide has the launch's)
     uint32_t nunits;           // care classes
     uint32_t tile_stride;
     const uint32_t *table;
@@ -105,6 +106,72 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
             }
         }
     }
+}
+
+// Separator bitmap of a staged tile, for tables with guides of extra
+// padding: bit b of word w stands for genome position t0 - 32 * kHaloL +
+// 32 * w + b (the positions of the staged planes) and is set where the
+// genome holds a separator (a poison position) or ends (positions below 0
+// and from g.total on).  Thread t writes words t, t + kThreads, ... (read
+// after the tile pipeline's first barrier).
+constexpr uint32_t kSepBytes = kLdsWords * 4;
+__device__ __forceinline__ void stage_separators(const DevGenome &g, uint64_t tile, uint32_t *s_sep) {
+    const int64_t base = (int64_t)tile * kTilePos - 32 * kHaloL;
+    const uint64_t lo = base > 0 ? (uint64_t)base : 0, hi = (uint64_t)(base + 32 * kLdsWords);
+    const int t = (int)threadIdx.x;
+    uint32_t m[kStageIters] = {};
+    uint64_t i = g.tile_pidx[tile ? tile - 1 : 0];  // first separator at or after the previous tile's start
+    for (;; i++) {
+        const uint64_t x = g.poison[i];  // poison[npoison] is the sentinel
+        if (x >= hi) break;
+        if (x < lo) continue;
+        const uint32_t b = (uint32_t)((int64_t)x - base), w = b >> 5;
+#pragma unroll
+        for (int k = 0; k < kStageIters; k++)
+            if (w == (uint32_t)(t + k * kThreads)) m[k] |= 1u << (b & 31);
+    }
+#pragma unroll
+    for (int k = 0; k < kStageIters; k++) {
+        const int w = t + k * kThreads;
+        const int64_t ws = base + 32 * w;  // position of bit 0
+        if (ws < 0) m[k] |= ws <= -32 ? ~0u : (1u << -ws) - 1;
+        const int64_t te = (int64_t)g.total - ws;  // first bit at or past the genome end
+        if (te < 32) m[k] |= te <= 0 ? ~0u : ~((1u << te) - 1);
+        if (w < kLdsWords) s_sep[w] = m[k];
+    }
+}
+
+// The sink of the pair (candidate entry ce, guide with variant block vb);
+// false when the pair has no record.  A bulge table may hold queries of one
+// pattern with different DNA-bulge sizes D: the launch finds the windows of
+// the smallest padding (the pattern with the fewest padding N), and vb[1] is
+// the guide's extra padding d (read by the PAD instances, which run the
+// tables that have such guides; s_sep is then the tile's separator bitmap).  Where the padding
+// lies before the window in the genome (forward PAM on the + strand,
+// reversed PAM on the - strand) the guide's window starts d positions
+// earlier; on the other side it ends d positions later.  The pair has no
+// record when those d positions leave the genome or hold a separator
+// (cas-offinder.cl:64-67 for the longer pattern): one lookup in the
+// bitmap, branch-free (the callers hold many live registers here).
+template <bool PAD>
+__device__ __forceinline__ bool bulge_sink(const BulgeArgs &a, uint64_t t0, uint32_t ce, kptr_t vb, StageRef st,
+                                           const uint32_t *s_sep, TileSink &out) {
+    const uint32_t off = ce >> 1, strand = ce & 1;
+    const uint64_t pos = t0 + off;
+    if (!PAD) {
+        out = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, pos, strand, vb[0]}, st, stage_low(strand, off)};
+        return true;
+    }
+    const uint32_t d = vb[1];
+    const bool before = (strand == 0) != (a.rev != 0);
+    // first padding position, as a bit of the separator bitmap (in
+    // [32 * kHaloL - kMaxShifts, 32 * (kLdsWords - 1)) for every candidate)
+    const uint32_t b = (before ? off - d : off + a.pd.P) + 32 * kHaloL;
+    const uint32_t sep = funnel(s_sep[(b >> 5) + 1], s_sep[b >> 5], b & 31) & ((1u << d) - 1);
+    const uint32_t sh = before ? d : 0u;
+    out = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, pos - sh, strand, vb[0]}, st,
+                   stage_low(strand, (int32_t)off - (int32_t)sh)};
+    return sep == 0;
 }
 
 // Window planes of candidate entry e (tile offset << 1 | strand) at the NS

@@ -287,8 +287,10 @@ __device__ __forceinline__ void emit_hit(const HitSink &o, uint32_t key, uint32_
 // Hits of a workgroup are staged in LDS and written out once per candidate
 // pass with one global atomic, so the hit counter sees one atomic per tile
 // pass instead of one per hit (same-address atomics serialize in one L2
-// channel).  An entry packs global key index (bits 23..46), mismatches
-// (16..22), strand (15) and the start's offset in the tile (0..14).
+// channel).  An entry packs global key index (bits 24..47), mismatches
+// (17..23), strand (16) and the window start's offset from the tile start
+// plus kStageBias (0..15): a bulge guide whose padding precedes the window
+// reports a start up to kMaxShifts - 1 positions before the tile.
 struct HitStage {  // header of a workgroup's stage
     uint32_t n;       // entries appended this pass; past the capacity they went to HBM directly
     uint32_t pad;
@@ -309,18 +311,26 @@ __device__ __forceinline__ StageRef stage_ref(HitStageBuf<CAP> &b) {
     return StageRef{&b.h, b.ent, (uint32_t)CAP};
 }
 constexpr int kStageCap = 512;  // default stage entries
-static_assert(kTilePos <= (1 << 15) && kMaxPattern < (1 << 7), "stage entry fields");
+constexpr int32_t kStageBias = 16;
+static_assert(kTilePos + kStageBias <= (1 << 16) && kMaxShifts <= kStageBias && kMaxPattern < (1 << 7),
+              "stage entry fields");
+
+// Low bits of a stage entry: strand and the window start's offset from the
+// tile start (at least -kStageBias).
+__device__ __forceinline__ uint32_t stage_low(uint32_t strand, int32_t off) {
+    return strand << 16 | (uint32_t)(off + kStageBias);
+}
 
 struct TileSink {
     HitSink g;      // direct path when the stage is full
     StageRef st;
-    uint32_t low;   // strand << 15 | tile offset
+    uint32_t low;   // stage_low(strand, window start - tile start)
 };
 
 __device__ __forceinline__ void emit_hit(const TileSink &o, uint32_t key, uint32_t mm) {
     const uint32_t i = atomicAdd(&o.st.h->n, 1u);
     if (i < o.st.cap)
-        o.st.ent[i] = ((uint64_t)(o.g.gbase + key) << 23) | (mm << 16) | o.low;
+        o.st.ent[i] = ((uint64_t)(o.g.gbase + key) << 24) | (mm << 17) | o.low;
     else
         emit_hit(o.g, key, mm);
 }
@@ -338,8 +348,8 @@ __device__ __forceinline__ void flush_stage(StageRef st, uint64_t t0, uint64_t *
         const uint64_t e = st.ent[i];
         const unsigned long long o = base + i;
         if (o < cap) {
-            hkey[o] = ((e >> 23) << kHitKeyShift) | ((t0 + (e & 0x7FFF)) << 1) | ((e >> 15) & 1);
-            hmm[o] = (uint16_t)((e >> 16) & 0x7F);
+            hkey[o] = ((e >> 24) << kHitKeyShift) | ((t0 + (e & 0xFFFF) - kStageBias) << 1) | ((e >> 16) & 1);
+            hmm[o] = (uint16_t)((e >> 17) & 0x7F);
         }
     }
     __syncthreads();

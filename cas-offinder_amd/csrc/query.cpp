@@ -595,39 +595,53 @@ static uint32_t low_mask(int32_t x, int w) {
     return y <= 0 ? 0u : y >= 32 ? ~0u : (1u << y) - 1u;
 }
 
+std::string unpadded_pattern(const Query &q) {
+    return q.rev ? q.pattern.substr(0, q.P - q.D) : q.pattern.substr(q.D);
+}
+
 int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
     t = BulgeTable();
     if (qs.empty()) return 0;
+    // The queries share the pattern without its padding and the PAM side;
+    // their bulge sizes may differ.  The table's alignments are the union:
+    // shift 0, DNA bulges 1..Dmax, RNA bulges 1..Rmax (shift Dmax + size);
+    // the launch finds the windows of the query with the least padding
+    // (Dmin) and each guide records its extra padding (bulge_sink).
     const Query &q0 = *qs[0].q;
-    const uint32_t D = q0.D, R = q0.R;
-    const int ns = (int)(D + R + 1);
-    if (D + R == 0 || ns > kMaxShifts) return 0;
+    const std::string base = unpadded_pattern(q0);
+    uint32_t Dmin = q0.D, Dmax = 0, Rmax = 0;
+    const Query *qmin = &q0;
     uint64_t allcare = 0;
     bool gen = false;
     for (const BulgeRef &r : qs) {
         const Query &q = *r.q;
-        if (q.D != D || q.R != R || q.rev != q0.rev || q.pattern != q0.pattern) return 0;
+        if (q.D + q.R == 0 || q.rev != q0.rev || unpadded_pattern(q) != base) return 0;
         if (q.bulge.empty() && !q.keys.empty()) return 0;
+        if (q.D < Dmin) Dmin = q.D, qmin = &q;
+        Dmax = std::max(Dmax, q.D);
+        Rmax = std::max(Rmax, q.R);
         for (const BulgeGuide &bg : q.bulge) {
             allcare |= bg.care;
             gen |= !bg.concrete;
         }
     }
+    const int ns = (int)(Dmax + Rmax + 1);
+    if (ns > kMaxShifts) return 0;
     const int gcb = allcare ? __builtin_ctzll(allcare) : 0;
     const int top = allcare ? 63 - __builtin_clzll(allcare) : 0;
     const int nw = top - gcb + 1 <= 32 ? 1 : 2;
     // window alignment of each shift: the left or right part of a variant
     // moves by the bulge size (cas-offinder.cpp:650-682 key layouts)
-    const int sx = q0.rev ? 0 : (int)D;
+    const int sx = q0.rev ? 0 : (int)Dmin;
     for (int u = 0; u < ns; u++) {
         int delta = 0;
-        if (u >= 1 && u <= (int)D) delta = q0.rev ? u : -u;                  // DNA bulge of size u
-        else if (u > (int)D) delta = q0.rev ? -(u - (int)D) : u - (int)D;    // RNA bulge of size u - D
+        if (u >= 1 && u <= (int)Dmax) delta = q0.rev ? u : -u;                    // DNA bulge of size u
+        else if (u > (int)Dmax) delta = q0.rev ? -(u - (int)Dmax) : u - (int)Dmax;  // RNA bulge of size u - Dmax
         t.cb[u] = sx + delta + gcb;
-        t.gap[u] = u > (int)D ? (uint32_t)(u - (int)D) : 0u;
+        t.gap[u] = u > (int)Dmax ? (uint32_t)(u - (int)Dmax) : 0u;
         // every LDS word the kernel reads for a window start in the tile stays
         // inside the staged tile + halo, on both strands
-        const int64_t P = q0.P, halo = 32 * kHaloL;
+        const int64_t P = qmin->P, halo = 32 * kHaloL;
         const int64_t plus_lo = t.cb[u] + halo, plus_hi = kTilePos - 1 + t.cb[u] + halo + 32 * (nw - 1);
         const int64_t minus_lo = P - 32 * nw - t.cb[u] + halo;
         const int64_t minus_hi = kTilePos - 1 + P - 32 * nw - t.cb[u] + halo + 32 * (nw - 1);
@@ -638,12 +652,15 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
     t.gen = gen;
     t.ns = ns;
     t.rev = q0.rev;
-    t.pat = q0.pat;
+    t.pat = qmin->pat;
+    t.dmin = Dmin;
+    t.dpad = Dmax - Dmin;
     auto word = [&](uint64_t x, int w) { return (uint32_t)((x >> gcb) >> (32 * w)); };
     // care classes (one class of all-ones care for general tables)
     struct Item {
         const BulgeGuide *bg;
         uint32_t gbase;
+        const Query *q;
     };
     std::map<std::vector<uint32_t>, std::vector<Item>> cls;
     std::vector<std::vector<uint32_t>> order;
@@ -653,8 +670,8 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
             if (!gen)
                 for (int w = 0; w < nw; w++) c[w] = word(bg.care, w);
             auto it = cls.find(c);
-            if (it == cls.end()) { order.push_back(c); cls[c].push_back({&bg, r.gbase}); }
-            else it->second.push_back({&bg, r.gbase});
+            if (it == cls.end()) { order.push_back(c); cls[c].push_back({&bg, r.gbase, r.q}); }
+            else it->second.push_back({&bg, r.gbase, r.q});
         }
     const int KP = bulge_planes(nw, gen), S = bulge_stride(nw, gen, ns);
     size_t rec_end = 0;
@@ -670,6 +687,18 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
         p += kClassHdr;
         for (const Item &it : items) {
             const BulgeGuide &bg = *it.bg;
+            // table shift u -> the guide's own family (its query's layout:
+            // DNA sizes 1..D, then RNA sizes 1..R), -1 where it has none;
+            // and where each own family's entries start in bg.ent
+            const int Dq = (int)it.q->D, Rq = (int)it.q->R;
+            int srcu[kMaxShifts];
+            size_t eoff[kMaxShifts + 1] = {0};
+            for (int v = 0; v <= Dq + Rq; v++) eoff[v + 1] = eoff[v] + 2 * (size_t)bg.n[v];
+            for (int u = 0; u < ns; u++) {
+                if (u == 0) srcu[u] = 0;
+                else if (u <= (int)Dmax) srcu[u] = u <= Dq ? u : -1;
+                else srcu[u] = u - (int)Dmax <= Rq ? Dq + (u - (int)Dmax) : -1;
+            }
             uint32_t *rec = &out[p];
             if (gen) {
                 for (int c5 = 0; c5 < 5; c5++)
@@ -681,23 +710,29 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
                     rec[2 * nw + w] = word(bg.care, w);
                 }
             }
-            for (int u = 0; u < ns; u++)
-                rec[KP + 1 + u] = bg.tmax[u] < 0 ? kNeverHit : (uint32_t)(-(int64_t)bg.tmax[u] - (int64_t)t.gap[u] - 1);
-            // variant block (rec is not used past this point: out grows)
+            for (int u = 0; u < ns; u++) {
+                const int v = srcu[u];
+                rec[KP + 1 + u] = v < 0 || bg.tmax[v] < 0 ? kNeverHit
+                                                          : (uint32_t)(-(int64_t)bg.tmax[v] - (int64_t)t.gap[u] - 1);
+            }
+            // variant block (rec is not used past this point: out grows):
+            // [gbase, extra padding, 0, 0], then one family block per shift
             const size_t vb = out.size();
             rec[KP] = (uint32_t)vb;
             const int FB = bulge_family_words(nw), SW = bulge_seg_words(nw);
             out.resize(vb + 4 + (size_t)ns * FB, 0);
             out[vb] = it.gbase;
+            out[vb + 1] = it.q->D - Dmin;
             const size_t e = out.size();
-            size_t src = 0;
             for (int u = 0; u < ns; u++) {
+                const int v = srcu[u];
+                if (v < 0) continue;  // no family here: zero block, never passes its bias
                 uint32_t *fb = &out[vb + 4 + (size_t)u * FB];
-                const int32_t jb0 = bg.j0[u] - gcb;
-                const uint32_t n = bg.n[u], nseg = std::min<uint32_t>(n, kBulgeSegs);
+                const int32_t jb0 = bg.j0[v] - gcb;
+                const uint32_t n = bg.n[v], nseg = std::min<uint32_t>(n, kBulgeSegs);
                 fb[0] = (uint32_t)jb0;
                 fb[1] = n;
-                fb[2] = (uint32_t)(e + src);
+                fb[2] = (uint32_t)(e + eoff[v]);
                 fb[3] = nseg;
                 // segment s holds splits [a, a + cnt); its lower bound counts
                 // `left` below the first split, `left & right` inside and
@@ -714,7 +749,6 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
                         sr[2 + 2 * nw + w] = ~low_mask(jz, w);
                     }
                 }
-                src += 2 * (size_t)n;
             }
             out.insert(out.end(), bg.ent.begin(), bg.ent.end());
             p += S;
@@ -780,7 +814,10 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
     d.slot_off = d.nwords * d.wstride;
     if (!slice_desc_valid(d, ns)) return false;  // e.g. care runs shorter than 13 positions
     // count bound of each filter family per guide record (-1: the family
-    // never emits); the no-bulge key's threshold joins family 0
+    // never emits); the no-bulge key's threshold joins the guide's first
+    // family (popc(M_0 & M_u) <= popc(M_0) for every u), family 0 when it
+    // has none, so that a guide's families stay its own (a table may merge
+    // queries of different bulge sizes)
     std::vector<std::array<int, 4>> bound(n);
     for (uint32_t r = 0; r < n; r++) {
         const uint32_t *rec = &t.words[kClassHdr + (size_t)r * S];
@@ -788,16 +825,29 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
             const uint32_t b = rec[KP + 1 + u];
             return b == kNeverHit ? -1 : (int)(-(int32_t)b - 1);
         };
+        int first = 0;
+        while (first < nf && tof(u_of[first]) < 0) first++;
+        if (first == nf) first = 0;
         for (int f = 0; f < nf; f++) {
             int T = tof(u_of[f]);
-            if (f == 0) T = std::max(T, tof(0));
+            if (f == first) T = std::max(T, tof(0));
             bound[r][f] = std::min(T, (1 << kSliceTPlanes) - 1);
         }
     }
     auto top = [&](uint32_t r) { return *std::max_element(bound[r].begin(), bound[r].begin() + nf); };
+    // slots grouped by the set of families their guides have (a table may
+    // merge queries of different bulge sizes; the kernel skips a family no
+    // guide of a word has), then by decreasing bound
+    auto fams = [&](uint32_t r) {
+        uint32_t m = 0;
+        for (int f = 0; f < nf; f++) m |= (bound[r][f] >= 0 ? 1u : 0u) << f;
+        return m;
+    };
     std::vector<uint32_t> order(n);
     for (uint32_t r = 0; r < n; r++) order[r] = r;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) { return top(x) > top(y); });
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+        return fams(x) != fams(y) ? fams(x) > fams(y) : top(x) > top(y);
+    });
     s.words.assign(d.slot_off + (size_t)kBulgeSlotWords * n, 0);
     for (uint32_t si = 0; si < n; si++) {  // slot si holds guide record order[si]
         const uint32_t r = order[si];

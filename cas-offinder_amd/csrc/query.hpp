@@ -133,7 +133,9 @@ struct BulgeTable {
     int32_t cb[kMaxShifts] = {};     // window position of bit 0 at each shift
     uint32_t gap[kMaxShifts] = {};   // RNA-bulge length of each shift's family
     bool rev = false;
-    PatternDesc pat{};  // finder pattern of the group
+    PatternDesc pat{};  // finder pattern of the group: the query with the least padding
+    uint32_t dmin = 0;  // that query's DNA-bulge size (the padding every guide's own D extends)
+    uint32_t dpad = 0;  // largest extra padding of a guide (largest D - dmin)
     uint64_t nguides = 0, nkeys = 0, care_sum = 0;
     int tmax = 0;  // largest threshold of an emitting variant
 };
@@ -175,6 +177,12 @@ struct BulgeRef {
     uint32_t gbase;  // global index of the query's first key
 };
 // 1: built; 0: the queries do not fit the bulge-family kernel (use key tables).
+// The pattern line of q without the bulge padding (D leading N for a
+// forward PAM, D trailing N for a reversed one).
+std::string unpadded_pattern(const Query &q);
+// Bulge table of queries of one unpadded pattern and PAM side (their bulge
+// sizes may differ; 1 + largest D + largest R <= kMaxShifts): 1 on success,
+// 0 when the queries do not qualify (their keys then go to key tables).
 int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t);
 
 }  // namespace cof

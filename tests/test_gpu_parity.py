@@ -173,10 +173,11 @@ def test_batched_multi_query_matches_single(ctx, tmp_path):
         assert H.record_lines(rb.format(i)) == exp
 
 
-def _planted_guides(r, recs, pattern, core, n, iupac=False):
+def _planted_guides(r, recs, pattern, core, n, iupac=False, edge=False):
     """Guides copied from windows that pass the PAM (either strand), with a
     planted DNA bulge (a guide base deleted), RNA bulge (a base inserted) and
-    up to two substitutions, so that the bulge families have sites to find."""
+    up to two substitutions, so that the bulge families have sites to find.
+    edge: every other window within five positions of a sequence end."""
     P = len(pattern)
     rev = pattern[0] != "N"
     rpat = H.revcomp(pattern)
@@ -184,6 +185,8 @@ def _planted_guides(r, recs, pattern, core, n, iupac=False):
     while len(out) < n:
         _, s = r.choice(recs)
         x = r.randrange(0, len(s) - P)
+        if edge and len(out) % 2:
+            x = r.choice([r.randint(0, 5), max(0, len(s) - P - r.randint(0, 5))])
         w = s[x:x + P]
         if H.window_passes(rpat, w):
             w = H.revcomp(w)
@@ -420,3 +423,57 @@ def test_mask_index_ten_and_twelve_ranges(tmp_path, parts, n_guides, tmax, rever
                 os.environ[k] = v
     assert len(exp) > 0
     assert any(", 3, %s, " % parts in k for k in kernels), kernels  # the 10- or 12-row kernel ran
+
+
+MERGED_CASES = [
+    # (pattern, guide core length, (D, R) of the queries sharing the pattern)
+    ("N" * 21 + "RG", 20, [(0, 1), (1, 0), (1, 1), (2, 0), (0, 2), (2, 2), (0, 0)]),
+    ("N" * 20 + "NGG", 20, [(1, 1), (0, 1), (1, 0)]),
+    ("TTTV" + "N" * 23, 23, [(1, 0), (0, 1), (2, 1), (1, 2)]),
+    ("N" * 20 + "NGG", 20, [(3, 1), (0, 2), (1, 0)]),  # union of shifts past five: one launch per (D, R)
+]
+
+
+@pytest.mark.parametrize("case", range(len(MERGED_CASES)))
+def test_merged_bulge_sizes(ctx, tmp_path, case):
+    """Queries of one pattern with different DNA/RNA bulge sizes searched in
+    one call share a bulge launch (the windows of the least padding; each
+    guide moves its window start by its own extra padding and needs those
+    positions inside the genome and free of ';').  On short contigs, with
+    ';' inside sequence lines and N runs, so that many windows touch a
+    contig edge or a separator, every query's records equal the oracle's
+    for that query alone."""
+    pattern, core, drs = MERGED_CASES[case]
+    r = random.Random(1300 + case)
+    recs = []
+    for i in range(40):  # about 100 kbp: several tiles, separators near their edges
+        seq = list(H.random_seq(r, r.randint(150, 4800), "ACGT", n_runs=r.randint(0, 2)))
+        for _ in range(r.randint(0, 2)):
+            seq[r.randrange(len(seq))] = ";"
+        recs.append(("s%d" % i, "".join(seq)))
+    gp = str(tmp_path / "m.fa")
+    H.write_fasta(gp, recs)
+    clean = [(n, s.replace(";", "A")) for n, s in recs]
+    rev = pattern[0] != "N"
+    tail = "N" * (len(pattern) - core)
+    texts = []
+    for qi, (D, R) in enumerate(drs):
+        cores = _planted_guides(r, clean, pattern, core, 6, edge=True)
+        guides = [tail + c if rev else c + tail for c in cores]
+        head = "%s %d %d" % (pattern, D, R) if D + R else pattern
+        lines = [head] + ["%s %d q%dg%d" % (g, r.randint(1, 5), qi, i) for i, g in enumerate(guides)]
+        texts.append(gp + "\n" + "\n".join(lines) + "\n")
+    ctx.load(gp)
+    qs = [ctx.query(t) for t in texts]
+    res = ctx.search(qs, casoff.MODE_BATCHED)
+    bulge_launches = [rec for rec in ctx.launch_info() if rec["sig"].startswith("bulge")]
+    n = 0
+    for qi, t in enumerate(texts):
+        exp = H.record_lines(H.oracle_records(tmp_path, t, name="q%d.txt" % qi))
+        got = H.record_lines(res.format(qi))
+        assert got == exp, "case %d query %d (D, R) = %s: %d vs %d records" % (case, qi, drs[qi], len(got), len(exp))
+        n += len(exp)
+    assert n > 0
+    nb = sum(1 for D, R in drs if D + R)
+    merged = max(D for D, _ in drs) + max(R for _, R in drs) + 1 <= 5
+    assert len(bulge_launches) == (1 if merged else nb), bulge_launches
