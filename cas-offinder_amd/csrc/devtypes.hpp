@@ -46,9 +46,11 @@ constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 // Bulge-family tables (device-side DNA/RNA-bulge expansion): a guide is scored
 // at kMaxShifts window alignments at most (its own plus one per bulge family).
 // Records, grouped by care class like concrete key tables ([care[NW], nguides,
-// 0, 0] headers), are [guide planes][variant block offset][bias per shift],
+// 0, 0] headers), are [guide planes][variant block word][bias per shift],
 // padded to 4 dwords; planes are glo[NW], ghi[NW], care[NW] (concrete) or the
-// five class masks [class * NW + word] (general).  A variant block is [gbase, 0, 0, 0]
+// five class masks [class * NW + word] (general).  The variant block word is
+// the block's offset | the guide's extra padding << kVbPadShift (tables that
+// merge bulge sizes).  A variant block is [gbase, 0, 0, 0]
 // followed by one family block per shift: [first split (bit coordinate,
 // int32), split count, entry offset, segment count] and kBulgeSegs segment
 // records [first split index, split count, below[NW], inner[NW], above[NW]]
@@ -56,13 +58,15 @@ constexpr int key_bias_index(int nw, bool gen) { return (gen ? 5 : 2) * nw; }
 // query-local key index] of every split.
 constexpr int kMaxShifts = 5;
 constexpr int kBulgeSegs = 4;
+constexpr int kVbPadShift = 29;
+constexpr uint32_t kVbOffsetMask = (1u << kVbPadShift) - 1;
 constexpr int bulge_planes(int nw, bool gen) { return gen ? 5 * nw : 3 * nw; }
 constexpr int bulge_stride(int nw, bool gen, int ns) { return (bulge_planes(nw, gen) + 1 + ns + 3) & ~3; }
 constexpr int bulge_seg_words(int nw) { return 2 + 3 * nw; }
 constexpr int bulge_family_words(int nw) { return 4 + kBulgeSegs * bulge_seg_words(nw); }
 constexpr uint32_t kNeverHit = 1u << 30;  // bias of a record or entry that never hits
 // Bulge seed index (bulge_index_kernel): MaskDesc rows, then per slot
-// [glo, ghi, variant block offset, family filter bias of each alignment].
+// [glo, ghi, variant block word, family filter bias of each alignment].
 constexpr int kBulgeSlotWords = 3 + kMaxShifts;
 constexpr uint32_t kBulgeIndexMaxBytes = 32 * 1024;  // static LDS + index stays below 64 KiB
 

@@ -27,7 +27,6 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
     uint32_t *qrec = s_qrec[wave], *qinfo = s_qinfo[wave], *qm = s_qm[wave];
     const kptr_t tab = (kptr_t)a.table;
     extern __shared__ __align__(16) uint32_t s_sep[];  // separator bitmap, kLdsWords words (PAD)
-    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     tile_candidates<1, true>(a.g, a.pd, a.limit, a.tile_stride, a.cand_count, s_lo, s_hi, s_ot, s_list, s_wsum,
                              stage_ref(s_stage), a.hkey, a.hmm, a.count, a.cap,
                              [&](uint64_t t0, auto rd, const uint32_t (&e)[1], const bool (&valid)[1]) {
@@ -40,14 +39,16 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
                 const uint32_t info = qinfo[lane];
                 const kptr_t kp = tab + qrec[lane];
                 const uint32_t ce = info & 0xFFFFu, fam = info >> 16;
+                const uint32_t room = pad_room<PAD>(a, ce, s_sep);
                 uint32_t M[NS][NW];
 #pragma unroll
                 for (int u = 0; u < NS; u++)
 #pragma unroll
                     for (int w = 0; w < NW; w++) M[u][w] = qm[lane * NS * NW + u * NW + w];
-                const kptr_t vb = tab + kp[KP];
-                TileSink sink;
-                const bool ok = bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink);
+                const uint32_t vw = kp[KP];
+                const kptr_t vb = tab + (vw & kVbOffsetMask);
+                const TileSink sink = bulge_sink<PAD>(a, t0, ce, tab, vw, stage_ref(s_stage));
+                const bool ok = !PAD || (vw >> kVbPadShift) <= room;
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if (ok && ((fam >> u) & 1)) {
@@ -68,6 +69,7 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
 
         uint32_t l[NS][NW], h[NS][NW], o[NS][NW];
         bulge_windows<NW, NS>(a, rd, e[0], l, h, o);
+        const uint32_t room = pad_room<PAD>(a, e[0], s_sep);
         uint32_t cls[GEN ? NS : 1][5][NW];
         if (GEN) {
 #pragma unroll
@@ -123,10 +125,10 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
                 const bool pass = valid[0] && acc < 0;
                 if (!QUEUE) {
                     if (pass) {
-                        const kptr_t vb = tab + kp[KP];
-                        const uint32_t ce = e[0];
-                        TileSink sink;
-                        const bool ok = bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink);
+                        const uint32_t vw = kp[KP];
+                        const kptr_t vb = tab + (vw & kVbOffsetMask);
+                        const TileSink sink = bulge_sink<PAD>(a, t0, e[0], tab, vw, stage_ref(s_stage));
+                        const bool ok = !PAD || (vw >> kVbPadShift) <= room;
 #pragma unroll
                         for (int u = 0; u < NS; u++) {
                             if (ok && f[u] < 0) {
@@ -169,6 +171,8 @@ __global__ __launch_bounds__(kThreads, NW == 1 ? (QUEUE ? 5 : 6) : 1) void bulge
             }
         }
         if (QUEUE && qn) drain(qn);
+    }, 0u, nullptr, NoFin{}, [&] {
+        if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     });
 }
 

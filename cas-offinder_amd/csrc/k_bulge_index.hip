@@ -30,7 +30,6 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
     extern __shared__ __align__(16) uint32_t s_idx[];
     const BulgeArgs &a = ia.b;
     uint32_t *s_sep = s_idx + ia.md.nwords;  // separator bitmap, kLdsWords words (PAD)
-    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     const MaskDesc &md = ia.md;
     for (uint32_t i = threadIdx.x; i < md.nwords / 4; i += kThreads)
         reinterpret_cast<uint4 *>(s_idx)[i] = reinterpret_cast<const uint4 *>(ia.idx)[i];
@@ -79,6 +78,7 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
             oc[u] = o[u][0] & care;
         }
         const uint32_t ce = e[0];
+        const uint32_t room = pad_room<PAD>(a, ce, s_sep);
 #pragma unroll
         for (int w = 0; w < NK; w++) {
             for (uint32_t x = sel[w]; x; x &= x - 1) {
@@ -98,10 +98,9 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
                     fl[u] = (int32_t)(bias[u] + (uint32_t)__popc(u == 0 ? M[0][0] : M[u][0] & M[0][0]));
                     acc |= fl[u];
                 }
-                if (acc >= 0) continue;
-                const kptr_t vb = tab + s0.z;
-                TileSink sink;
-                if (!bulge_sink<PAD>(a, t0, ce, vb, stage_ref(s_stage), s_sep, sink)) continue;
+                if (acc >= 0 || (PAD && (s0.z >> kVbPadShift) > room)) continue;
+                const kptr_t vb = tab + (s0.z & kVbOffsetMask);
+                const TileSink sink = bulge_sink<PAD>(a, t0, ce, tab, s0.z, stage_ref(s_stage));
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if (fl[u] < 0) {
@@ -112,6 +111,8 @@ __global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(
                 }
             }
         }
+    }, 0u, nullptr, NoFin{}, [&] {
+        if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     });
 }
 

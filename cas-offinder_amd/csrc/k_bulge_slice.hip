@@ -58,7 +58,6 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
     const BulgeArgs &a = sa.b;
     const SliceDesc &sd = sa.sd;
     uint32_t *s_sep = s_slot + sd.nslots * kBulgeSlotWords;  // separator bitmap, kLdsWords words (PAD)
-    if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
     for (uint32_t i = threadIdx.x; i < sd.nslots * kBulgeSlotWords / 4; i += kThreads)
         reinterpret_cast<uint4 *>(s_slot)[i] = reinterpret_cast<const uint4 *>(sa.slices + sd.slot_off)[i];
     // visible after tile_candidates' first barrier
@@ -78,6 +77,7 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
         if (lane < n) {
             const uint32_t item = q[lane];
             const uint32_t slot = item >> 16, ce = item & 0xFFFFu;
+            const uint32_t room = pad_room<PAD>(a, ce, s_sep);
             uint32_t l[NS][1], h[NS][1], o[NS][1];
             bulge_windows<1, NS>(a, rd, ce, l, h, o);
             const uint4 s0 = slot4[2 * slot], s1 = slot4[2 * slot + 1];
@@ -96,9 +96,9 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
                 fl[u] = (int32_t)(bias[u] + (uint32_t)__popc(u == 0 ? M[0] : M[u] & M[0]));
                 acc |= fl[u];
             }
-            TileSink sink;
-            if (acc < 0 && bulge_sink<PAD>(a, t0, ce, tab + s0.z, stage_ref(s_stage), s_sep, sink)) {
-                const kptr_t vb = tab + s0.z;
+            if (acc < 0 && (!PAD || (s0.z >> kVbPadShift) <= room)) {
+                const TileSink sink = bulge_sink<PAD>(a, t0, ce, tab, s0.z, stage_ref(s_stage));
+                const kptr_t vb = tab + (s0.z & kVbOffsetMask);
 #pragma unroll
                 for (int u = 0; u < NS; u++) {
                     if (fl[u] < 0) {
@@ -222,6 +222,9 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
         [&](uint64_t t0, auto rd) {
             if (qn) drain(t0, rd, qn);
             qn = 0;
+        },
+        [&] {
+            if (PAD) stage_separators(a.g, (uint64_t)blockIdx.x * a.tile_stride, s_sep);
         });
 }
 

@@ -141,37 +141,44 @@ __device__ __forceinline__ void stage_separators(const DevGenome &g, uint64_t ti
     }
 }
 
-// The sink of the pair (candidate entry ce, guide with variant block vb);
-// false when the pair has no record.  A bulge table may hold queries of one
-// pattern with different DNA-bulge sizes D: the launch finds the windows of
-// the smallest padding (the pattern with the fewest padding N), and vb[1] is
-// the guide's extra padding d (read by the PAD instances, which run the
-// tables that have such guides; s_sep is then the tile's separator bitmap).  Where the padding
-// lies before the window in the genome (forward PAM on the + strand,
-// reversed PAM on the - strand) the guide's window starts d positions
-// earlier; on the other side it ends d positions later.  The pair has no
-// record when those d positions leave the genome or hold a separator
-// (cas-offinder.cl:64-67 for the longer pattern): one lookup in the
-// bitmap, branch-free (the callers hold many live registers here).
+// A bulge table may hold queries of one pattern with different DNA-bulge
+// sizes D: the launch finds the windows of the smallest padding (the
+// pattern with the fewest padding N), and a guide of a larger D carries the
+// extra padding d in its variant block word (vw >> kVbPadShift; the PAD
+// instances run those tables, with the tile's separator bitmap s_sep).
+// Where the padding lies before the window in the genome (forward PAM on the
+// + strand, reversed PAM on the - strand) the guide's window starts d
+// positions earlier; on the other side it ends d positions later.  The
+// pair has no record when those d positions leave the genome or hold a
+// separator (cas-offinder.cl:64-67 for the longer pattern).
+//
+// pad_room: free positions (at most kMaxShifts) beside candidate ce's window
+// on the padding side, from one funnel of the bitmap: the pair of a guide of
+// extra padding d has a record only if d <= room.
 template <bool PAD>
-__device__ __forceinline__ bool bulge_sink(const BulgeArgs &a, uint64_t t0, uint32_t ce, kptr_t vb, StageRef st,
-                                           const uint32_t *s_sep, TileSink &out) {
+__device__ __forceinline__ uint32_t pad_room(const BulgeArgs &a, uint32_t ce, const uint32_t *s_sep) {
+    if (!PAD) return 0;
     const uint32_t off = ce >> 1, strand = ce & 1;
-    const uint64_t pos = t0 + off;
-    if (!PAD) {
-        out = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, pos, strand, vb[0]}, st, stage_low(strand, off)};
-        return true;
-    }
-    const uint32_t d = vb[1];
     const bool before = (strand == 0) != (a.rev != 0);
-    // first padding position, as a bit of the separator bitmap (in
-    // [32 * kHaloL - kMaxShifts, 32 * (kLdsWords - 1)) for every candidate)
-    const uint32_t b = (before ? off - d : off + a.pd.P) + 32 * kHaloL;
-    const uint32_t sep = funnel(s_sep[(b >> 5) + 1], s_sep[b >> 5], b & 31) & ((1u << d) - 1);
-    const uint32_t sh = before ? d : 0u;
-    out = TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, pos - sh, strand, vb[0]}, st,
-                   stage_low(strand, (int32_t)off - (int32_t)sh)};
-    return sep == 0;
+    // bit 0: the first of the kMaxShifts positions before / after the window
+    // (bitmap bits [32 * kHaloL - kMaxShifts, 32 * (kLdsWords - 1)) for every candidate)
+    const uint32_t b = (before ? off - kMaxShifts : off + a.pd.P) + 32 * kHaloL;
+    const uint32_t x = funnel(s_sep[(b >> 5) + 1], s_sep[b >> 5], b & 31) & ((1u << kMaxShifts) - 1);
+    // before: positions nearest the window are the high bits; after: the low ones
+    return before ? (uint32_t)__clz(x) - (32 - kMaxShifts) : (uint32_t)__builtin_ctz(x | 1u << kMaxShifts);
+}
+
+// The sink of the pair (candidate entry ce, guide whose variant block word
+// is vw = block offset | extra padding << kVbPadShift): the record starts at
+// the guide's own window start.
+template <bool PAD>
+__device__ __forceinline__ TileSink bulge_sink(const BulgeArgs &a, uint64_t t0, uint32_t ce, kptr_t tab, uint32_t vw,
+                                               StageRef st) {
+    const uint32_t off = ce >> 1, strand = ce & 1;
+    const uint32_t gbase = tab[vw & kVbOffsetMask];
+    const uint32_t sh = PAD && (strand == 0) != (a.rev != 0) ? vw >> kVbPadShift : 0u;
+    return TileSink{HitSink{a.hkey, a.hmm, a.count, a.cap, t0 + off - sh, strand, gbase}, st,
+                    stage_low(strand, (int32_t)off - (int32_t)sh)};
 }
 
 // Window planes of candidate entry e (tile offset << 1 | strand) at the NS

@@ -562,18 +562,25 @@ constexpr int kListCap = 4096;  // LDS candidate entries per pass
 // candidate passes, so that every barrier is reached by all of them.
 // s_nmax: G words of LDS (G > 1 only).
 // fin(t0, rd) runs on every thread after the candidates of each pass and
-// before the stage is flushed (per-wave queues drain there).
+// before the stage is flushed (per-wave queues drain there).  pre() runs on
+// every thread while the tile's plane loads are in flight (before the
+// first barrier: LDS it writes is visible to the candidates).
 struct NoFin {
     template <typename Rd>
     __device__ void operator()(uint64_t, Rd) const {}
 };
-template <int W, bool ALL = false, int LC = kListCap, int G = 1, typename Cands, typename Fin = NoFin>
+struct NoPre {
+    __device__ void operator()() const {}
+};
+template <int W, bool ALL = false, int LC = kListCap, int G = 1, typename Cands, typename Fin = NoFin,
+          typename Pre = NoPre>
 __device__ __forceinline__ void tile_candidates(const DevGenome &g, const PatternDesc &pd, uint64_t limit,
                                                 uint32_t tile_stride, unsigned long long *cand_count,
                                                 uint32_t *s_lo, uint32_t *s_hi, uint32_t *s_ot, uint16_t *s_list,
                                                 uint32_t *s_wsum, StageRef st, uint64_t *hkey, uint16_t *hmm,
                                                 unsigned long long *count, uint64_t cap, Cands &&cands,
-                                                uint32_t ntile = 0, uint32_t *s_nmax = nullptr, Fin &&fin = Fin{}) {
+                                                uint32_t ntile = 0, uint32_t *s_nmax = nullptr, Fin &&fin = Fin{},
+                                                Pre &&pre = Pre{}) {
     const int t = G == 1 ? (int)threadIdx.x : (int)(threadIdx.x & (kThreads - 1));
     const uint32_t grp = G == 1 ? 0u : threadIdx.x / kThreads;
     const uint32_t gi = G == 1 ? blockIdx.x : blockIdx.x * G + grp;
@@ -581,7 +588,12 @@ __device__ __forceinline__ void tile_candidates(const DevGenome &g, const Patter
     const uint32_t tile = live ? gi * tile_stride : 0u;
     const int64_t w0 = (int64_t)tile * kTileWords;
     const uint64_t ppos = g.tile_ppos[tile];
-    stage_tile(g, w0, s_lo, s_hi, s_ot, t);
+    {
+        TileRegs r;
+        load_tile(g, w0, r, t);
+        pre();
+        store_tile(r, s_lo, s_hi, s_ot, t);
+    }
     if (t == 0) st.h->n = 0;
     __syncthreads();
     const uint64_t t0 = (uint64_t)w0 * 32, t1 = t0 + kTilePos;

@@ -716,13 +716,14 @@ int build_bulge_table(const std::vector<BulgeRef> &qs, BulgeTable &t) {
                                                           : (uint32_t)(-(int64_t)bg.tmax[v] - (int64_t)t.gap[u] - 1);
             }
             // variant block (rec is not used past this point: out grows):
-            // [gbase, extra padding, 0, 0], then one family block per shift
+            // [gbase, 0, 0, 0], then one family block per shift; the record
+            // holds its offset and the guide's extra padding
             const size_t vb = out.size();
-            rec[KP] = (uint32_t)vb;
+            if (vb > kVbOffsetMask) return 0;
+            rec[KP] = (uint32_t)vb | (it.q->D - Dmin) << kVbPadShift;
             const int FB = bulge_family_words(nw), SW = bulge_seg_words(nw);
             out.resize(vb + 4 + (size_t)ns * FB, 0);
             out[vb] = it.gbase;
-            out[vb + 1] = it.q->D - Dmin;
             const size_t e = out.size();
             for (int u = 0; u < ns; u++) {
                 const int v = srcu[u];
