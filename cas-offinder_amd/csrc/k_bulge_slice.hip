@@ -26,7 +26,9 @@
 // A genome N reads as base T here (no non-ACGT plane): a count can
 // only drop, so the filter stays a lower bound.  The (candidate, guide)
 // pairs that pass go to a per-wave LDS queue and are counted exactly, one
-// pair per lane, with bulge_kernel's family filters and split counts.
+// pair per lane, with bulge_kernel's family filters and every split of a
+// passing family counted in one loop (bulge_family_flat: the lanes hold
+// different guides, so segment bounds would diverge).
 #include <type_traits>
 
 #include "kbulge.hpp"
@@ -104,7 +106,7 @@ __global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
                     if (fl[u] < 0) {
                         const bool lft0 = u == 0 || a.rev;
                         const uint32_t left[1] = {lft0 ? M[0] : M[u]}, right[1] = {lft0 ? M[u] : M[0]};
-                        bulge_family<1>(tab, vb + 4 + FB * u, left, right, a.gap[u], bias[u], sink);
+                        bulge_family_flat<1>(tab, vb + 4 + FB * u, left, right, a.gap[u], bias[u], sink);
                     }
                 }
             }

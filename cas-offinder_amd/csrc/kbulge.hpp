@@ -108,6 +108,26 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
     }
 }
 
+// The variants of one family, every split counted in turn: for lanes that
+// each hold their own (candidate, guide) pair (bulge_slice_kernel's queue),
+// where the segment bounds of bulge_family diverge across the lanes and the
+// wave runs every lane's path; here the lanes share one loop of at most the
+// family's split count.  A split's entry is read only when its count is
+// within the family's largest threshold (bias + gap + mm < 0).
+template <int NW>
+__device__ __forceinline__ void bulge_family_flat(kptr_t tab, kptr_t fb, const uint32_t (&left)[NW],
+                                                  const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
+                                                  const TileSink &sink) {
+    const int32_t j0 = (int32_t)fb[0];
+    const uint32_t n = fb[1];
+    const kptr_t ent = tab + fb[2];
+    for (uint32_t k = 0; k < n; k++) {
+        const int32_t j = j0 + (int32_t)k;
+        const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
+        if ((int32_t)(bias + gap + mm) < 0 && (int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+    }
+}
+
 // Separator bitmap of a staged tile, for tables with guides of extra
 // padding: bit b of word w stands for genome position t0 - 32 * kHaloL +
 // 32 * w + b (the positions of the staged planes) and is set where the
