@@ -112,8 +112,18 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
 // each hold their own (candidate, guide) pair (bulge_slice_kernel's queue),
 // where the segment bounds of bulge_family diverge across the lanes and the
 // wave runs every lane's path; here the lanes share one loop of at most the
-// family's split count.  A split's entry is read only when its count is
-// within the family's largest threshold (bias + gap + mm < 0).
+// family's split count.  The count of split j + 1 is the count of split j
+// plus left's bit j minus right's bit j + gap; a split's entry is read only
+// when its count is within the family's largest threshold (bias + gap + mm
+// < 0).
+template <int NW>
+__device__ __forceinline__ uint32_t mask_bit(const uint32_t (&m)[NW], int32_t x) {
+    uint32_t b = 0;
+#pragma unroll
+    for (int w = 0; w < NW; w++)
+        if ((uint32_t)(x - 32 * w) < 32u) b = (m[w] >> (x - 32 * w)) & 1u;
+    return b;
+}
 template <int NW>
 __device__ __forceinline__ void bulge_family_flat(kptr_t tab, kptr_t fb, const uint32_t (&left)[NW],
                                                   const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
@@ -121,10 +131,12 @@ __device__ __forceinline__ void bulge_family_flat(kptr_t tab, kptr_t fb, const u
     const int32_t j0 = (int32_t)fb[0];
     const uint32_t n = fb[1];
     const kptr_t ent = tab + fb[2];
+    uint32_t mm = popc_below<NW>(left, j0) + popc_from<NW>(right, j0 + (int32_t)gap);
+    const uint32_t bg = bias + gap;
     for (uint32_t k = 0; k < n; k++) {
         const int32_t j = j0 + (int32_t)k;
-        const uint32_t mm = popc_below<NW>(left, j) + popc_from<NW>(right, j + (int32_t)gap);
-        if ((int32_t)(bias + gap + mm) < 0 && (int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+        if ((int32_t)(bg + mm) < 0 && (int32_t)(ent[2 * k] + mm) < 0) emit_hit(sink, ent[2 * k + 1], mm);
+        mm += mask_bit<NW>(left, j) - mask_bit<NW>(right, j + (int32_t)gap);
     }
 }
 
