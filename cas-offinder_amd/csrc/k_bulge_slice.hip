@@ -47,8 +47,13 @@ constexpr int kSliceQueue = 64;
 // DN / DP: window shifts -1..-DN and +1..+DP of the filter families (the
 // alignments of the table other than 0); KN: care positions of the guide
 // words (the table's kn is in (KN - 4, KN]).
+// Three workgroups per CU where the registers allow it without spilling
+// (care runs of up to 24 positions, 28 with one family).
+template <int DN, int DP, int KN>
+constexpr int slice_min_blocks() { return KN <= 24 || (DN + DP == 1 && KN <= 28) ? 3 : 1; }
+
 template <int DN, int DP, int KN, bool PAD>
-__global__ __launch_bounds__(kThreads) void bulge_slice_kernel(SliceArgs sa) {
+__global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bulge_slice_kernel(SliceArgs sa) {
     constexpr int NF = DN + DP, NS = NF + 1, OFF = DN > DP ? DN : DP, PN = KN + 2 * OFF;
     constexpr int FW = PN <= 32 ? 1 : 2;  // frame words of the broadcast window positions
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
