@@ -13,7 +13,7 @@ namespace cof {
 // per family u the rows at 0 and u are OR-ed and a guide survives when, for
 // some family, at least its word's need of them hold it.  Only survivors get
 // the family filters and exact counts of bulge_kernel (in place).  Index (LDS): rows of nk words per (range, code),
-// then per slot [glo, ghi, variant block offset, family filter bias per
+// then per slot [glo, ghi, variant block word, family filter bias per
 // alignment] (kBulgeSlotWords words).
 struct BulgeIndexArgs {
     BulgeArgs b;
@@ -21,8 +21,10 @@ struct BulgeIndexArgs {
     const uint32_t *idx;
 };
 
+// Register budget: six waves per SIMD for one-word rows (they fit 80
+// VGPRs without spilling), five for two, three for four.
 template <int NK, int NS, int NP, bool PAD>
-__global__ __launch_bounds__(kThreads, NK == 4 ? 3 : 5) void bulge_index_kernel(BulgeIndexArgs ia) {
+__global__ __launch_bounds__(kThreads, NK == 4 ? 3 : NK == 2 ? 5 : 6) void bulge_index_kernel(BulgeIndexArgs ia) {
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
     __shared__ uint16_t s_list[kListCap];
     __shared__ uint32_t s_wsum[4];

@@ -34,12 +34,13 @@ struct MaskArgs {
 
 // G tiles per workgroup (groups of kThreads threads) share one copy of the
 // index in LDS; with G > 1 the candidate list holds kListCap entries.
-// Register budget: six waves per SIMD, or three for wide rows (NK >=
-// kMaskWideWords), whose LDS index allows no more waves anyway; with per-word
+// Register budget: six waves per SIMD, five for 12-word rows of up to 10
+// ranges (they fit 96 VGPRs without spilling), or three for wider rows,
+// whose LDS index allows no more waves anyway; with per-word
 // counts their rows are read and counted for all words first, so that the
 // reads of every chunk issue before the survivor loops.
 template <int NK, int MINEX, int NP, int G>
-__global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? 3 : 6) void fused_mask_kernel(MaskArgs a) {
+__global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? (NK <= 12 && NP <= 10 ? 5 : 3) : 6) void fused_mask_kernel(MaskArgs a) {
     constexpr int LC = G == 1 ? kSeedListCap : kListCap;
     __shared__ uint32_t s_lo[G][kLdsWords], s_hi[G][kLdsWords], s_ot[G][kLdsWords];
     __shared__ uint16_t s_list[G][LC];
