@@ -426,11 +426,13 @@ def test_mask_index_ten_and_twelve_ranges(tmp_path, parts, n_guides, tmax, rever
 
 
 MERGED_CASES = [
-    # (pattern, guide core length, (D, R) of the queries sharing the pattern)
-    ("N" * 21 + "RG", 20, [(0, 1), (1, 0), (1, 1), (2, 0), (0, 2), (2, 2), (0, 0)]),
-    ("N" * 20 + "NGG", 20, [(1, 1), (0, 1), (1, 0)]),
-    ("TTTV" + "N" * 23, 23, [(1, 0), (0, 1), (2, 1), (1, 2)]),
-    ("N" * 20 + "NGG", 20, [(3, 1), (0, 2), (1, 0)]),  # union of shifts past five: one launch per (D, R)
+    # (pattern, guide core length, (D, R) of the queries sharing the pattern, IUPAC codes in guides)
+    ("N" * 21 + "RG", 20, [(0, 1), (1, 0), (1, 1), (2, 0), (0, 2), (2, 2), (0, 0)], False),
+    ("N" * 20 + "NGG", 20, [(1, 1), (0, 1), (1, 0)], False),
+    ("TTTV" + "N" * 23, 23, [(1, 0), (0, 1), (2, 1), (1, 2)], False),
+    ("N" * 20 + "NGG", 20, [(3, 1), (0, 2), (1, 0)], False),  # union of shifts past five: one launch per (D, R)
+    ("N" * 21 + "RG", 20, [(1, 1), (0, 1), (2, 0)], True),  # general (IUPAC) table
+    ("N" * 34 + "NGG", 34, [(1, 0), (0, 1), (2, 1)], False),  # care positions past 32: two-word masks
 ]
 
 
@@ -443,7 +445,7 @@ def test_merged_bulge_sizes(ctx, tmp_path, case):
     ';' inside sequence lines and N runs, so that many windows touch a
     contig edge or a separator, every query's records equal the oracle's
     for that query alone."""
-    pattern, core, drs = MERGED_CASES[case]
+    pattern, core, drs, iupac = MERGED_CASES[case]
     r = random.Random(1300 + case)
     recs = []
     for i in range(40):  # about 100 kbp: several tiles, separators near their edges
@@ -458,7 +460,7 @@ def test_merged_bulge_sizes(ctx, tmp_path, case):
     tail = "N" * (len(pattern) - core)
     texts = []
     for qi, (D, R) in enumerate(drs):
-        cores = _planted_guides(r, clean, pattern, core, 6, edge=True)
+        cores = _planted_guides(r, clean, pattern, core, 6, iupac=iupac, edge=True)
         guides = [tail + c if rev else c + tail for c in cores]
         head = "%s %d %d" % (pattern, D, R) if D + R else pattern
         lines = [head] + ["%s %d q%dg%d" % (g, r.randint(1, 5), qi, i) for i, g in enumerate(guides)]
