@@ -42,7 +42,10 @@ struct SliceArgs {
     const uint32_t *slices;  // word blocks, then slot records (sd.slot_off)
 };
 
-constexpr int kSliceQueue = 64;
+// Per-wave queue of passing (candidate, guide) pairs: drained 64 at a time
+// (one per lane) whenever it holds that many, the rest at the end of each
+// candidate pass, so that drains inside the guide-word loop run full waves.
+constexpr int kSliceQueue = 128;
 
 // DN / DP: window shifts -1..-DN and +1..+DP of the filter families (the
 // alignments of the table other than 0); KN: care positions of the guide
@@ -205,15 +208,12 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                     pass |= le & tp[kSliceTPlanes];
                 }
                 pass &= live;
-                // queue the passing (candidate, guide) pairs, one per lane per round
+                // queue the passing (candidate, guide) pairs, one per lane per
+                // round (qn < 64 before a round, so the queue holds its np <= 64)
                 for (;;) {
                     const uint64_t m = __ballot(pass != 0);
                     if (!m) break;
                     const uint32_t np = (uint32_t)__popcll(m);
-                    if (qn + np > (uint32_t)kSliceQueue) {
-                        drain(t0, rd, qn);
-                        qn = 0;
-                    }
                     if (pass) {
                         const uint32_t g = (uint32_t)__builtin_ctz(pass);
                         pass &= pass - 1;
@@ -222,6 +222,12 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                         q[at] = ((32 * w + g) << 16) | ce;
                     }
                     qn += np;
+                    if (qn >= 64) {
+                        drain(t0, rd, 64);  // entries 0..63; the rest moves to the front
+                        const uint32_t rest = qn - 64, v = lane < rest ? q[64 + lane] : 0u;
+                        if (lane < rest) q[lane] = v;
+                        qn = rest;
+                    }
                 }
             }
         },
