@@ -345,7 +345,7 @@ def main():
     ap.add_argument("--no-plain", action="store_true", help="skip the plain path (profiling runs)")
     ap.add_argument("--serve-requests", type=int, default=60, help="requests of the serving run (0: skip)")
     ap.add_argument("--serve-qps", type=float, default=20.0)
-    ap.add_argument("--serve-sweep", default="20,40,80,120,160,240,320",
+    ap.add_argument("--serve-sweep", default="20,40,80,120,160,200,240,280,320",
                     help="offered request rates of the served-capacity sweep ('' to skip)")
     ap.add_argument("--serve-p99-ms", type=float, default=100.0, help="p99 latency bound of the served capacity")
     ap.add_argument("--serve-plain", type=int, default=12, help="requests the plain path serves beside the sweep")

@@ -13,6 +13,7 @@
 // average of hits per compare key of the batches served so far; a batch that
 // finds more hits than its lane holds grows the lane's buffers and reruns
 // (counted in cof_server_stats).
+#include <algorithm>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -123,29 +124,45 @@ void cof_server::worker(Lane *L) {
         const auto ts1 = clk::now();
         const bool grew = L->d_hkey.n > cap_before;
         const uint64_t found = rc == COF_OK ? res->hits.size() : 0;
-        // split the merged result per request, query indices rebased to 0
+        // split the merged result per request, query indices rebased to 0 (a
+        // batch of one request keeps the merged result); each request's
+        // records are copied by host threads, query run by query run
         std::vector<cof_result *> parts(batch.size(), nullptr);
-        if (rc == COF_OK) {
+        if (rc == COF_OK && batch.size() == 1) {
+            res->genome_serial = ctx->genome_serial;  // reloads are refused while the server runs
+            parts[0] = res;
+            res = nullptr;
+        } else if (rc == COF_OK) {
             size_t q0 = 0;
             for (size_t i = 0; i < batch.size(); i++) {
                 auto *p = new cof_result();
-                p->genome_serial = ctx->genome_serial;  // reloads are refused while the server runs
+                p->genome_serial = ctx->genome_serial;
                 const size_t nqi = batch[i]->qs.size();
                 p->queries = batch[i]->qs;
                 p->qbegin.assign(nqi, 0);
                 p->qend.assign(nqi, 0);
-                size_t total = 0;
-                for (size_t j = 0; j < nqi; j++) total += res->qend[q0 + j] - res->qbegin[q0 + j];
-                p->hits.resize(total);
                 size_t at = 0;
                 for (size_t j = 0; j < nqi; j++) {
                     p->qbegin[j] = at;
-                    for (size_t h = res->qbegin[q0 + j]; h < res->qend[q0 + j]; h++, at++) {
-                        p->hits[at] = res->hits[h];
-                        p->hits[at].query = (uint8_t)j;
-                    }
+                    at += res->qend[q0 + j] - res->qbegin[q0 + j];
                     p->qend[j] = at;
                 }
+                p->hits.resize(at);
+                cof_hit *dst = p->hits.data();
+                const cof_hit *src = res->hits.data();
+                const std::vector<size_t> &b = p->qbegin, &e = p->qend;
+                const size_t *sb = res->qbegin.data() + q0;
+                parallel_for(at, 1 << 16, [&](uint64_t x0, uint64_t x1) {
+                    size_t j = (size_t)(std::upper_bound(e.begin(), e.end(), (size_t)x0) - e.begin());
+                    for (uint64_t x = x0; x < x1;) {
+                        const uint64_t end = std::min<uint64_t>(x1, e[j]);
+                        for (; x < end; x++) {
+                            dst[x] = src[sb[j] + (x - b[j])];
+                            dst[x].query = (uint8_t)j;
+                        }
+                        j++;
+                    }
+                });
                 parts[i] = p;
                 q0 += nqi;
             }
