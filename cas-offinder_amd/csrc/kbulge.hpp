@@ -1,4 +1,4 @@
-// Shared parts of bulge_kernel and bulge_index_kernel.
+// Shared parts of bulge_kernel, bulge_index_kernel and bulge_slice_kernel.
 #pragma once
 #include "kcommon.hpp"
 
@@ -108,14 +108,7 @@ __device__ __forceinline__ void bulge_family(kptr_t tab, kptr_t fb, const uint32
     }
 }
 
-// The variants of one family, every split counted in turn: for lanes that
-// each hold their own (candidate, guide) pair (bulge_slice_kernel's queue),
-// where the segment bounds of bulge_family diverge across the lanes and the
-// wave runs every lane's path; here the lanes share one loop of at most the
-// family's split count.  The count of split j + 1 is the count of split j
-// plus left's bit j minus right's bit j + gap; a split's entry is read only
-// when its count is within the family's largest threshold (bias + gap + mm
-// < 0).
+// bit x of an NW-word mask (0 for x outside it)
 template <int NW>
 __device__ __forceinline__ uint32_t mask_bit(const uint32_t (&m)[NW], int32_t x) {
     uint32_t b = 0;
@@ -124,6 +117,15 @@ __device__ __forceinline__ uint32_t mask_bit(const uint32_t (&m)[NW], int32_t x)
         if ((uint32_t)(x - 32 * w) < 32u) b = (m[w] >> (x - 32 * w)) & 1u;
     return b;
 }
+
+// The variants of one family, every split counted in turn: for lanes that
+// each hold their own (candidate, guide) pair (bulge_slice_kernel's queue),
+// where the segment bounds of bulge_family diverge across the lanes and the
+// wave runs every lane's path; here the lanes share one loop of at most the
+// family's split count.  The count of split j + 1 is the count of split j
+// plus left's bit j minus right's bit j + gap; a split's entry is read only
+// when its count is within the family's largest threshold (bias + gap + mm
+// < 0).
 template <int NW>
 __device__ __forceinline__ void bulge_family_flat(kptr_t tab, kptr_t fb, const uint32_t (&left)[NW],
                                                   const uint32_t (&right)[NW], uint32_t gap, uint32_t bias,
