@@ -29,8 +29,6 @@
 // pair per lane, with bulge_kernel's family filters and every split of a
 // passing family counted in one loop (bulge_family_flat: the lanes hold
 // different guides, so segment bounds would diverge).
-#include <type_traits>
-
 #include "kbulge.hpp"
 #include "slicecount.hpp"
 
@@ -51,9 +49,9 @@ constexpr int kSliceQueue = 128;
 // alignments of the table other than 0); KN: care positions of the guide
 // words (the table's kn is in (KN - 4, KN]).
 // Three workgroups per CU where the registers allow it without spilling
-// (care runs of up to 24 positions, 28 with one family).
+// (care runs of up to 24 positions), two for 28 with one family.
 template <int DN, int DP, int KN>
-constexpr int slice_min_blocks() { return KN <= 24 || (DN + DP == 1 && KN <= 28) ? 3 : 1; }
+constexpr int slice_min_blocks() { return KN <= 24 ? 3 : DN + DP == 1 && KN <= 28 ? 2 : 1; }
 
 template <int DN, int DP, int KN, bool PAD>
 __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bulge_slice_kernel(SliceArgs sa) {
@@ -151,6 +149,11 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                 }
                 uint32_t pass = 0;
                 const bool fine = (blk[2 * KN + (kSliceTPlanes + 1) * NF] & 1u) != 0;  // uniform
+                uint32_t c0[6] = {};
+                if (fine) {
+                    if (a.rev) slice_half_shared<KN, true>(M0, c0);
+                    else slice_half_shared<KN, false>(M0, c0);
+                }
 #pragma unroll
                 for (int f = 0; f < NF; f++) {
                     const int d = f < DN ? -(f + 1) : f - DN + 1;
@@ -171,39 +174,12 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                         slice_count<KN>(b, c);
                         le = slice_le(c, t);
                     } else {
-                        // splits up to care index S - 1, then splits from S: below the
-                        // second half's splits only the left part compares, from S + 1
-                        // (the largest gap is 2) on only the right part of the first
-                        // half's.  The left part is alignment u for a forward PAM and
-                        // alignment 0 for a reversed one.  Each half's count is the sum
-                        // of the counts over care positions [0, S) and [S, KN), so that
-                        // only half-width planes are live at once.
-                        constexpr int S = KN / 2;
-                        auto halves = [&](auto rvc) {
-                            constexpr bool RV = decltype(rvc)::value;
-                            uint32_t lo1[S], lo2[S], a1[6], a2[6], b1[6], b2[6], h1[6], h2[6];
-#pragma unroll
-                            for (int k = 0; k < S; k++) {
-                                const uint32_t m = mu(k);
-                                lo1[k] = M0[k] & m;
-                                lo2[k] = RV ? M0[k] : m;
-                            }
-                            slice_count<S>(lo1, a1);
-                            slice_count<S>(lo2, a2);
-                            uint32_t hi1[KN - S], hi2[KN - S];
-#pragma unroll
-                            for (int k = S; k < KN; k++) {
-                                const uint32_t m = mu(k), b = M0[k] & m;
-                                hi1[k - S] = k == S ? b : RV ? m : M0[k];
-                                hi2[k - S] = b;
-                            }
-                            slice_count<KN - S>(hi1, b1);
-                            slice_count<KN - S>(hi2, b2);
-                            slice_add(a1, b1, h1);
-                            slice_add(a2, b2, h2);
-                            return slice_le(h1, t) | slice_le(h2, t);
-                        };
-                        le = a.rev ? halves(std::true_type{}) : halves(std::false_type{});
+                        // each half of the family's split range bounded on its own
+                        // (slice_halves), M_0's count shared by the families in c0
+                        uint32_t h1[6], h2[6];
+                        if (a.rev) slice_halves<KN, true>(M0, mu, sd.kn, c0, h1, h2);
+                        else slice_halves<KN, false>(M0, mu, sd.kn, c0, h1, h2);
+                        le = slice_le(h1, t) | slice_le(h2, t);
                     }
                     pass |= le & tp[kSliceTPlanes];
                 }

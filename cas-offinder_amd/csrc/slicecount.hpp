@@ -88,10 +88,10 @@ COF_SLICE_FN void slice_count(uint32_t (&v)[N], uint32_t (&c)[6]) {
     for (int w = 0; w < 6; w++) c[w] = P.out[w] >= 0 ? v[P.out[w] >= 0 ? P.out[w] : 0] : 0u;
 }
 
-// z = x + y, bit-sliced (six planes; the sum must stay below 64): a ripple
-// of full adders.
-COF_SLICE_FN void slice_add(const uint32_t (&x)[6], const uint32_t (&y)[6], uint32_t (&z)[6]) {
-    uint32_t carry = 0;
+// z = x + y + cin, bit-sliced (six planes, cin one plane; the sum must stay
+// below 64): a ripple of full adders.
+COF_SLICE_FN void slice_add(const uint32_t (&x)[6], const uint32_t (&y)[6], uint32_t (&z)[6], uint32_t cin = 0) {
+    uint32_t carry = cin;
 #pragma unroll
     for (int w = 0; w < 6; w++) {
         const uint32_t a = x[w], b = y[w];
@@ -105,16 +105,99 @@ COF_SLICE_FN void slice_add(const uint32_t (&x)[6], const uint32_t (&y)[6], uint
     }
 }
 
-// Lanes whose count c is at most t (both six bit planes): the count exceeds
-// t at the highest plane where they differ with c's bit set.
+// Lanes whose count c is at most t (both six bit planes): c <= t exactly
+// when c + (63 - t) carries nothing out of bit 5, so one carry chain of
+// majorities of c, ~t and the carry (one v_bitop3 per bit) decides it.
 COF_SLICE_FN uint32_t slice_le(const uint32_t (&c)[6], const uint32_t (&t)[6]) {
-    uint32_t gt = 0, eq = ~0u;
+    uint32_t carry = c[0] & ~t[0];
 #pragma unroll
-    for (int w = 5; w >= 0; w--) {
-        gt |= eq & c[w] & ~t[w];
-        eq &= ~(c[w] ^ t[w]);
+    for (int w = 1; w < 6; w++) {
+#if defined(__HIP_DEVICE_COMPILE__)
+        carry = __builtin_amdgcn_bitop3_b32(c[w], t[w], carry, 0xB2);  // majority of c, ~t, carry
+#else
+        carry = (c[w] & ~t[w]) | (carry & (c[w] | ~t[w]));
+#endif
     }
-    return ~gt;
+    return ~carry;
+}
+
+// ------------------------------------------------ half bounds of a bulge family
+//
+// A variant of a bulge family at split j counts the left part's mismatches
+// at care positions below j and the right part's from j + gap (gap <= 2;
+// cas-offinder.cpp:650-682), so with b = M_0 & M_u (a mismatch at both
+// alignments) every position counts at least b there, except the gap
+// positions (the filter bound adds gap to the threshold).  The splits of
+// the family are bounded on two halves, S = KN / 2:
+//   splits j < S: positions up to S hold b, those past S only the right part
+//     (j + gap <= S + 1): h1 = count(b, k <= S) + count(right, k > S);
+//   splits j >= S: positions below S only the left part:
+//     h2 = count(left, k < S) + count(b, k >= S).
+// The left part is M_u and the right part M_0 for a forward PAM (RV false),
+// the other way round for a reversed one, so one of the two counts of M_0
+// alone is the same for every family of a word: slice_half_shared computes
+// it once (forward: M_0 over (S, KN); reversed: M_0 over [0, S)).
+// Positions from kn on (KN - 4 < kn <= KN) lie past the guide's care run and
+// count nothing: m0 is zero there, and mu(k) is masked where it enters a
+// count alone.
+template <int KN, bool RV>
+COF_SLICE_FN void slice_half_shared(const uint32_t (&m0)[KN], uint32_t (&c0)[6]) {
+    constexpr int S = KN / 2;
+    if constexpr (RV) {
+        uint32_t v[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) v[k] = m0[k];
+        slice_count<S>(v, c0);
+    } else {
+        uint32_t v[KN - S - 1];
+#pragma unroll
+        for (int k = S + 1; k < KN; k++) v[k - S - 1] = m0[k];
+        slice_count<KN - S - 1>(v, c0);
+    }
+}
+
+// h1 / h2: count planes of the two half bounds of the family whose mismatch
+// plane at care position k is mu(k); c0 from slice_half_shared.
+template <int KN, bool RV, typename Mu>
+COF_SLICE_FN void slice_halves(const uint32_t (&m0)[KN], Mu mu, uint32_t kn, const uint32_t (&c0)[6],
+                               uint32_t (&h1)[6], uint32_t (&h2)[6]) {
+    constexpr int S = KN / 2;
+    uint32_t pa[S + 1], a1[6];  // b at k <= S
+    uint32_t x1[6] = {};        // forward: left = M_u below S
+    {
+        uint32_t px[S];
+#pragma unroll
+        for (int k = 0; k < S; k++) {
+            const uint32_t m = mu(k);
+            pa[k] = m0[k] & m;
+            px[k] = m;
+        }
+        if constexpr (!RV) slice_count<S>(px, x1);
+    }
+    const uint32_t bs = m0[S] & mu(S);
+    pa[S] = bs;
+    slice_count<S + 1>(pa, a1);
+    uint32_t pe[KN - S - 1], e1[6];  // b at k > S
+    uint32_t y1[6] = {};             // reversed: right = M_u past S
+    {
+        uint32_t py[KN - S - 1];
+#pragma unroll
+        for (int k = S + 1; k < KN; k++) {
+            uint32_t m = mu(k);
+            pe[k - S - 1] = m0[k] & m;
+            if (k >= KN - 3 && (uint32_t)k >= kn) m = 0;  // past the care run (kn > KN - 4)
+            py[k - S - 1] = m;
+        }
+        if constexpr (RV) slice_count<KN - S - 1>(py, y1);
+    }
+    slice_count<KN - S - 1>(pe, e1);
+    if constexpr (RV) {
+        slice_add(a1, y1, h1);
+        slice_add(c0, e1, h2, bs);
+    } else {
+        slice_add(a1, c0, h1);
+        slice_add(x1, e1, h2, bs);
+    }
 }
 
 }  // namespace cof

@@ -1,7 +1,11 @@
 // Host check of the bit-sliced counting network (cas-offinder_amd/csrc/
 // slicecount.hpp): for every plane count N in 1..32 and random planes, the
 // six count planes equal each lane's popcount, slice_le equals the per-lane
-// comparison with random bounds and slice_add the per-lane sum.  Prints "ok" or the first failure.
+// comparison with random bounds and slice_add the per-lane sum (with and
+// without a carry-in plane).  The half bounds of a bulge family
+// (slice_half_shared + slice_halves) equal their per-lane formula and are
+// at most every split's mismatch count plus the gap, with random planes
+// past the care run.  Prints "ok" or the first failure.
 #include <cstdio>
 #include <random>
 #include <utility>
@@ -29,8 +33,10 @@ static bool check(std::mt19937_64 &rng) {
             for (int w = 0; w < 6; w++) t[w] |= ((tv[g] >> w) & 1u) << g;
         }
         const uint32_t le = slice_le(c, t);
-        uint32_t sum[6];
+        uint32_t sum[6], sumc[6];
         slice_add(c, t, sum);
+        const uint32_t cin = (uint32_t)rng();
+        slice_add(c, t, sumc, cin);
         for (int g = 0; g < 32; g++) {
             int want = 0, got = 0;
             for (int i = 0; i < N; i++) want += (keep[i] >> g) & 1;
@@ -43,6 +49,13 @@ static bool check(std::mt19937_64 &rng) {
             for (int w = 0; w < 6; w++) s2 |= (int)((sum[w] >> g) & 1) << w;
             if (want + (int)tv[g] < 64 && s2 != want + (int)tv[g]) {
                 printf("add N=%d lane %d: %d + %u gave %d\n", N, g, want, tv[g], s2);
+                return false;
+            }
+            int s3 = 0;
+            for (int w = 0; w < 6; w++) s3 |= (int)((sumc[w] >> g) & 1) << w;
+            const int wc = want + (int)tv[g] + (int)((cin >> g) & 1);
+            if (wc < 64 && s3 != wc) {
+                printf("add+cin N=%d lane %d: gave %d, want %d\n", N, g, s3, wc);
                 return false;
             }
             if ((int)((le >> g) & 1) != (want <= (int)tv[g])) {
@@ -59,9 +72,79 @@ static bool all(std::mt19937_64 &rng, std::integer_sequence<int, N...>) {
     return (check<N + 1>(rng) && ...);
 }
 
+
+// per-lane count of planes p over care positions [lo, hi)
+template <int KN>
+static int lane_count(const uint32_t (&p)[KN], int g, int lo, int hi) {
+    int n = 0;
+    for (int k = lo; k < hi; k++) n += (p[k] >> g) & 1;
+    return n;
+}
+
+template <int KN, bool RV>
+static bool check_halves(std::mt19937_64 &rng) {
+    constexpr int S = KN / 2;
+    for (int rep = 0; rep < 3000; rep++) {
+        const int kn = KN - (int)(rng() % 4);  // care run of KN - 3 .. KN positions
+        uint32_t m0[KN], mu[KN];
+        for (int k = 0; k < KN; k++) {
+            // mismatch planes about as dense as a random window's (3 in 4), some sparse
+            uint32_t x = (uint32_t)rng() | (uint32_t)rng(), y = (uint32_t)rng() | (uint32_t)rng();
+            if (rep % 3 == 0) x &= (uint32_t)rng() & (uint32_t)rng();
+            if (rep % 5 == 0) y &= (uint32_t)rng() & (uint32_t)rng();
+            m0[k] = k < kn ? x : 0u;  // M_0 is zero past the care run
+            mu[k] = y;                // M_u holds whatever the window has there
+        }
+        uint32_t c0[6], h1[6], h2[6];
+        slice_half_shared<KN, RV>(m0, c0);
+        slice_halves<KN, RV>(m0, [&](int k) { return mu[k]; }, (uint32_t)kn, c0, h1, h2);
+        uint32_t mu_in[KN], b[KN];
+        for (int k = 0; k < KN; k++) {
+            mu_in[k] = k < kn ? mu[k] : 0u;
+            b[k] = m0[k] & mu[k];
+        }
+        const uint32_t(&left)[KN] = RV ? m0 : mu_in;
+        const uint32_t(&right)[KN] = RV ? mu_in : m0;
+        for (int g = 0; g < 32; g++) {
+            int v1 = 0, v2 = 0;
+            for (int w = 0; w < 6; w++) {
+                v1 |= (int)((h1[w] >> g) & 1) << w;
+                v2 |= (int)((h2[w] >> g) & 1) << w;
+            }
+            const int f1 = lane_count<KN>(b, g, 0, S + 1) + lane_count<KN>(right, g, S + 1, kn);
+            const int f2 = lane_count<KN>(left, g, 0, S) + lane_count<KN>(b, g, S, kn);
+            if (v1 != f1 || v2 != f2) {
+                printf("halves KN=%d RV=%d kn=%d lane %d: (%d, %d) vs formula (%d, %d)\n", KN, RV, kn, g, v1, v2,
+                       f1, f2);
+                return false;
+            }
+            // every split j of a family of gap 0..2: mm(j) + gap bounds the half holding j
+            for (int gap = 0; gap <= 2; gap++)
+                for (int j = 0; j + gap <= kn; j++) {
+                    const int mm = lane_count<KN>(left, g, 0, j) + lane_count<KN>(right, g, j + gap, kn);
+                    const int h = j < S ? v1 : v2;
+                    if (h > mm + gap) {
+                        printf("halves KN=%d RV=%d kn=%d lane %d gap %d split %d: bound %d > %d + gap\n", KN, RV, kn,
+                               g, gap, j, h, mm);
+                        return false;
+                    }
+                }
+        }
+    }
+    return true;
+}
+
+template <int KN>
+static bool check_halves_both(std::mt19937_64 &rng) {
+    return check_halves<KN, false>(rng) && check_halves<KN, true>(rng);
+}
+
 int main() {
     std::mt19937_64 rng(12345);
     if (!all(rng, std::make_integer_sequence<int, 32>{})) return 1;
+    if (!(check_halves_both<16>(rng) && check_halves_both<20>(rng) && check_halves_both<24>(rng) &&
+          check_halves_both<28>(rng) && check_halves_both<32>(rng)))
+        return 1;
     // op count of the networks the kernel uses: at most one full adder per input
     for (int n : {16, 20, 24, 28, 32}) {
         const CsaPlan p = csa_plan(n);

@@ -479,3 +479,56 @@ def test_merged_bulge_sizes(ctx, tmp_path, case):
     nb = sum(1 for D, R in drs if D + R)
     merged = max(D for D, _ in drs) + max(R for _, R in drs) + 1 <= 5
     assert len(bulge_launches) == (1 if merged else nb), bulge_launches
+
+
+HALVES_CASES = [
+    # (pattern, guide core length, D, R): care runs of 21 and 23 positions,
+    # so the slice filter's last guide word position lies past the run
+    ("TTTV" + "N" * 23, 23, 1, 1),
+    ("TTTV" + "N" * 23, 23, 0, 1),
+    ("N" * 21 + "NGG", 21, 1, 1),
+]
+
+
+@pytest.mark.parametrize("case", range(len(HALVES_CASES)))
+def test_bulge_halves_at_threshold(ctx, tmp_path, case):
+    """Bulge sites planted with exactly `threshold` substitutions (5 or 6)
+    and the bulge at every split of the guide: the bit-sliced filter bounds
+    such words on the two halves of each family's split range, and a site
+    whose count equals its threshold is found only if neither half
+    overcounts (positions past the care run count nothing).  Records of
+    plain and batched searches equal the oracle's."""
+    pattern, core, D, R = HALVES_CASES[case]
+    r = random.Random(1500 + case)
+    recs = [("c%d" % i, H.random_seq(r, r.randint(40000, 70000), "ACGT")) for i in range(4)]
+    gp = str(tmp_path / "h.fa")
+    H.write_fasta(gp, recs)
+    P = len(pattern)
+    rev = pattern[0] != "N"
+    rpat = H.revcomp(pattern)
+    tail = "N" * (P - core)
+    lines = ["%s %d %d" % (pattern, D, R)]
+    while len(lines) < 31:
+        _, s = r.choice(recs)
+        x = r.randrange(0, len(s) - P)
+        w = s[x:x + P]
+        if H.window_passes(rpat, w):
+            w = H.revcomp(w)
+        elif not H.window_passes(pattern, w):
+            continue
+        c = list(w[P - core:] if rev else w[:core])
+        split = 1 + (len(lines) * 7) % (core - 2)  # bulges spread over the guide
+        if len(lines) % 2 and R:  # the guide holds one base more than the genome
+            c.insert(split, r.choice("ACGT"))
+            c = c[:core] if rev else c[1:]
+        elif D:  # the genome holds one base more than the guide
+            del c[split]
+            c = c + [r.choice("ACGT")] if rev else [r.choice("ACGT")] + c
+        t = 5 + len(lines) % 2
+        for p in r.sample(range(core), t):
+            c[p] = r.choice([b for b in "ACGT" if b != c[p]])
+        g = "".join(c)
+        lines.append("%s %d g%d" % (tail + g if rev else g + tail, t, len(lines)))
+    text = gp + "\n" + "\n".join(lines) + "\n"
+    exp = check(ctx, tmp_path, gp, text, "halves case %d" % case)
+    assert len(exp) >= 15
