@@ -97,10 +97,13 @@ struct SliceDesc {
 };
 // What bulge_slice_kernel can run (the builder and the launcher both check
 // it): at most two filter families, one per alignment of a table of ns, a
-// care run of kn positions in (KN - 4, KN] for a kernel width KN of 16-32,
-// word blocks holding the planes and bounds, slots after the blocks.
+// care run of kn positions in (KN - 4, KN] for a kernel width KN of 16, 20,
+// 24, 28 or 32, or exactly KN = kn for the widths 21 and 23 (the kernel
+// masks the positions past a shorter run), word blocks holding the planes
+// and bounds, slots after the blocks.
 inline bool slice_desc_valid(const SliceDesc &sd, int ns) {
-    const bool width = sd.KN == 16 || sd.KN == 20 || sd.KN == 24 || sd.KN == 28 || sd.KN == 32;
+    const bool width = sd.KN == 16 || sd.KN == 20 || sd.KN == 24 || sd.KN == 28 || sd.KN == 32 ||
+                       ((sd.KN == 21 || sd.KN == 23) && sd.kn == sd.KN);
     return ns >= 2 && ns <= 3 && sd.dn <= 2 && sd.dp <= 2 && (int)(sd.dn + sd.dp) == ns - 1 && width &&
            sd.kn <= sd.KN && sd.kn + 4 > sd.KN && sd.nslots >= 1 && sd.nslots <= kSliceMaxGuides &&
            sd.nwords == (sd.nslots + 31) / 32 && sd.wstride % 4 == 0 &&

@@ -47,7 +47,9 @@ constexpr int kSliceQueue = 128;
 
 // DN / DP: window shifts -1..-DN and +1..+DP of the filter families (the
 // alignments of the table other than 0); KN: care positions of the guide
-// words (the table's kn is in (KN - 4, KN]).
+// words.  The round widths (16, 20, ..., 32) hold care runs of kn in
+// (KN - 4, KN] and mask the positions past the run; the widths 21 and 23
+// hold exactly their run (no mask).
 // Three workgroups per CU where the registers allow it without spilling
 // (care runs of up to 24 positions), two for 28 with one family.
 template <int DN, int DP, int KN>
@@ -55,6 +57,7 @@ constexpr int slice_min_blocks() { return KN <= 24 ? 3 : DN + DP == 1 && KN <= 2
 
 template <int DN, int DP, int KN, bool PAD>
 __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bulge_slice_kernel(SliceArgs sa) {
+    constexpr bool CUT = KN % 4 == 0;  // a round width: the care run may be shorter
     constexpr int NF = DN + DP, NS = NF + 1, OFF = DN > DP ? DN : DP, PN = KN + 2 * OFF;
     constexpr int FW = PN <= 32 ? 1 : 2;  // frame words of the broadcast window positions
     __shared__ uint32_t s_lo[kLdsWords], s_hi[kLdsWords], s_ot[kLdsWords];
@@ -145,7 +148,7 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
 #pragma unroll
                 for (int k = 0; k < KN; k++) {
                     M0[k] = __builtin_amdgcn_bitop3_b32(blk[k], WL[k + OFF], blk[KN + k] ^ WH[k + OFF], kXorOr);
-                    if (k >= KN - 3 && (uint32_t)k >= sd.kn) M0[k] = 0;  // positions past the care run
+                    if (CUT && k >= KN - 3 && (uint32_t)k >= sd.kn) M0[k] = 0;  // positions past the care run
                 }
                 uint32_t pass = 0;
                 const bool fine = (blk[2 * KN + (kSliceTPlanes + 1) * NF] & 1u) != 0;  // uniform
@@ -177,8 +180,8 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                         // each half of the family's split range bounded on its own
                         // (slice_halves), M_0's count shared by the families in c0
                         uint32_t h1[6], h2[6];
-                        if (a.rev) slice_halves<KN, true>(M0, mu, sd.kn, c0, h1, h2);
-                        else slice_halves<KN, false>(M0, mu, sd.kn, c0, h1, h2);
+                        if (a.rev) slice_halves<KN, true, CUT>(M0, mu, sd.kn, c0, h1, h2);
+                        else slice_halves<KN, false, CUT>(M0, mu, sd.kn, c0, h1, h2);
                         le = slice_le(h1, t) | slice_le(h2, t);
                     }
                     pass |= le & tp[kSliceTPlanes];
@@ -231,6 +234,8 @@ static hipError_t slice_kn(dim3 grid, const SliceArgs &a, uint32_t lds, hipStrea
     switch (a.sd.KN) {
         case 16: return slice_launch<DN, DP, 16>(grid, a, lds, s);
         case 20: return slice_launch<DN, DP, 20>(grid, a, lds, s);
+        case 21: return slice_launch<DN, DP, 21>(grid, a, lds, s);
+        case 23: return slice_launch<DN, DP, 23>(grid, a, lds, s);
         case 24: return slice_launch<DN, DP, 24>(grid, a, lds, s);
         case 28: return slice_launch<DN, DP, 28>(grid, a, lds, s);
         case 32: return slice_launch<DN, DP, 32>(grid, a, lds, s);

@@ -792,7 +792,9 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
         u_of[f] = u;
     }
     const int nf = dn + dp, off = std::max(dn, dp);
-    d.KN = kn <= 16 ? 16 : kn <= 20 ? 20 : kn <= 24 ? 24 : kn <= 28 ? 28 : 32;
+    // kernel width: the care run itself where a kernel of that width exists
+    // (21, 23), else the next of 16 ... 32 (positions past the run masked)
+    d.KN = kn == 21 || kn == 23 ? kn : kn <= 16 ? 16 : kn <= 20 ? 20 : kn <= 24 ? 24 : kn <= 28 ? 28 : 32;
     d.kn = (uint32_t)kn;
     d.dn = (uint32_t)dn;
     d.dp = (uint32_t)dp;

@@ -81,11 +81,11 @@ static int lane_count(const uint32_t (&p)[KN], int g, int lo, int hi) {
     return n;
 }
 
-template <int KN, bool RV>
+template <int KN, bool RV, bool CUT>
 static bool check_halves(std::mt19937_64 &rng) {
     constexpr int S = KN / 2;
     for (int rep = 0; rep < 3000; rep++) {
-        const int kn = KN - (int)(rng() % 4);  // care run of KN - 3 .. KN positions
+        const int kn = CUT ? KN - (int)(rng() % 4) : KN;  // care run of KN - 3 .. KN positions (CUT)
         uint32_t m0[KN], mu[KN];
         for (int k = 0; k < KN; k++) {
             // mismatch planes about as dense as a random window's (3 in 4), some sparse
@@ -97,7 +97,7 @@ static bool check_halves(std::mt19937_64 &rng) {
         }
         uint32_t c0[6], h1[6], h2[6];
         slice_half_shared<KN, RV>(m0, c0);
-        slice_halves<KN, RV>(m0, [&](int k) { return mu[k]; }, (uint32_t)kn, c0, h1, h2);
+        slice_halves<KN, RV, CUT>(m0, [&](int k) { return mu[k]; }, (uint32_t)kn, c0, h1, h2);
         uint32_t mu_in[KN], b[KN];
         for (int k = 0; k < KN; k++) {
             mu_in[k] = k < kn ? mu[k] : 0u;
@@ -114,8 +114,8 @@ static bool check_halves(std::mt19937_64 &rng) {
             const int f1 = lane_count<KN>(b, g, 0, S + 1) + lane_count<KN>(right, g, S + 1, kn);
             const int f2 = lane_count<KN>(left, g, 0, S) + lane_count<KN>(b, g, S, kn);
             if (v1 != f1 || v2 != f2) {
-                printf("halves KN=%d RV=%d kn=%d lane %d: (%d, %d) vs formula (%d, %d)\n", KN, RV, kn, g, v1, v2,
-                       f1, f2);
+                printf("halves KN=%d RV=%d CUT=%d kn=%d lane %d: (%d, %d) vs formula (%d, %d)\n", KN, RV, CUT, kn, g,
+                       v1, v2, f1, f2);
                 return false;
             }
             // every split j of a family of gap 0..2: mm(j) + gap bounds the half holding j
@@ -134,16 +134,18 @@ static bool check_halves(std::mt19937_64 &rng) {
     return true;
 }
 
-template <int KN>
+template <int KN, bool CUT>
 static bool check_halves_both(std::mt19937_64 &rng) {
-    return check_halves<KN, false>(rng) && check_halves<KN, true>(rng);
+    return check_halves<KN, false, CUT>(rng) && check_halves<KN, true, CUT>(rng);
 }
 
 int main() {
     std::mt19937_64 rng(12345);
     if (!all(rng, std::make_integer_sequence<int, 32>{})) return 1;
-    if (!(check_halves_both<16>(rng) && check_halves_both<20>(rng) && check_halves_both<24>(rng) &&
-          check_halves_both<28>(rng) && check_halves_both<32>(rng)))
+    // the kernel's widths: masked round widths, the exact widths 21 and 23
+    if (!(check_halves_both<16, true>(rng) && check_halves_both<20, true>(rng) && check_halves_both<24, true>(rng) &&
+          check_halves_both<28, true>(rng) && check_halves_both<32, true>(rng) && check_halves_both<21, false>(rng) &&
+          check_halves_both<23, false>(rng)))
         return 1;
     // op count of the networks the kernel uses: at most one full adder per input
     for (int n : {16, 20, 24, 28, 32}) {

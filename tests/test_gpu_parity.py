@@ -482,16 +482,18 @@ def test_merged_bulge_sizes(ctx, tmp_path, case):
 
 
 HALVES_CASES = [
-    # (pattern, guide core length, D, R): care runs of 21 and 23 positions,
-    # so the slice filter's last guide word position lies past the run
+    # (pattern, guide core length, D, R): care runs of 21 and 23 positions
+    # (slice kernel widths of their own) and 22 (width 24, the positions
+    # past the run masked)
     ("TTTV" + "N" * 23, 23, 1, 1),
     ("TTTV" + "N" * 23, 23, 0, 1),
     ("N" * 21 + "NGG", 21, 1, 1),
+    ("TTTV" + "N" * 22, 22, 1, 1),
 ]
 
 
 @pytest.mark.parametrize("case", range(len(HALVES_CASES)))
-def test_bulge_halves_at_threshold(ctx, tmp_path, case):
+def test_bulge_halves_at_threshold(ctx, tmp_path, case, request):
     """Bulge sites planted with exactly `threshold` substitutions (5 or 6)
     and the bulge at every split of the guide: the bit-sliced filter bounds
     such words on the two halves of each family's split range, and a site
@@ -532,3 +534,8 @@ def test_bulge_halves_at_threshold(ctx, tmp_path, case):
     text = gp + "\n" + "\n".join(lines) + "\n"
     exp = check(ctx, tmp_path, gp, text, "halves case %d" % case)
     assert len(exp) >= 15
+    if request.node.callspec.params["ctx"] == "3":  # the slice kernel ran, at the run's width
+        width = {21: 21, 22: 24, 23: 23}[core]
+        kernels = [rec["kernel"] for rec in ctx.launch_info()]
+        assert any(k.startswith("cof::bulge_slice_kernel<%d, %d, %d," % (D, R, width)) for k in kernels) or any(
+            k.startswith("cof::bulge_slice_kernel<%d, %d, %d," % (R, D, width)) for k in kernels), kernels
