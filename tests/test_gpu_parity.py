@@ -489,13 +489,17 @@ HALVES_CASES = [
     ("TTTV" + "N" * 23, 23, 0, 1),
     ("N" * 21 + "NGG", 21, 1, 1),
     ("TTTV" + "N" * 22, 22, 1, 1),
+    # bulges of two bases (RNA: a family of gap 2)
+    ("TTTV" + "N" * 23, 23, 0, 2),
+    ("N" * 20 + "NGG", 20, 0, 2),
+    ("N" * 21 + "NGG", 21, 2, 0),
 ]
 
 
 @pytest.mark.parametrize("case", range(len(HALVES_CASES)))
 def test_bulge_halves_at_threshold(ctx, tmp_path, case, request):
-    """Bulge sites planted with exactly `threshold` substitutions (5 or 6)
-    and the bulge at every split of the guide: the bit-sliced filter bounds
+    """Bulge sites (one or two bases) planted with exactly `threshold`
+    substitutions (5 or 6) and the bulge at every split of the guide: the bit-sliced filter bounds
     such words on the two halves of each family's split range, and a site
     whose count equals its threshold is found only if neither half
     overcounts (positions past the care run count nothing).  Records of
@@ -520,12 +524,14 @@ def test_bulge_halves_at_threshold(ctx, tmp_path, case, request):
             continue
         c = list(w[P - core:] if rev else w[:core])
         split = 1 + (len(lines) * 7) % (core - 2)  # bulges spread over the guide
-        if len(lines) % 2 and R:  # the guide holds one base more than the genome
-            c.insert(split, r.choice("ACGT"))
-            c = c[:core] if rev else c[1:]
-        elif D:  # the genome holds one base more than the guide
-            del c[split]
-            c = c + [r.choice("ACGT")] if rev else [r.choice("ACGT")] + c
+        if len(lines) % 2 and R:  # the guide holds R bases more than the genome
+            for _ in range(R):
+                c.insert(split, r.choice("ACGT"))
+            c = c[:core] if rev else c[R:]
+        elif D:  # the genome holds D bases more than the guide
+            del c[split:split + D]
+            pad = [r.choice("ACGT") for _ in range(D)]
+            c = c + pad if rev else pad + c
         t = 5 + len(lines) % 2
         for p in r.sample(range(core), t):
             c[p] = r.choice([b for b in "ACGT" if b != c[p]])
@@ -535,7 +541,7 @@ def test_bulge_halves_at_threshold(ctx, tmp_path, case, request):
     exp = check(ctx, tmp_path, gp, text, "halves case %d" % case)
     assert len(exp) >= 15
     if request.node.callspec.params["ctx"] == "3":  # the slice kernel ran, at the run's width
-        width = {21: 21, 22: 24, 23: 23}[core]
+        width = {20: 20, 21: 21, 22: 24, 23: 23}[core]
         kernels = [rec["kernel"] for rec in ctx.launch_info()]
         assert any(k.startswith("cof::bulge_slice_kernel<%d, %d, %d," % (D, R, width)) for k in kernels) or any(
             k.startswith("cof::bulge_slice_kernel<%d, %d, %d," % (R, D, width)) for k in kernels), kernels
