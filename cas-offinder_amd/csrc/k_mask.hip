@@ -108,6 +108,18 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? (NK <= 12 && N
                 uint32_t sel[NK];
 #pragma unroll
                 for (int c = 0; c < NK / 4; c++) {
+                    if constexpr (MINEX == 3) {
+                        // the chunk's four needs (4 bits each); all 15: no records
+                        uint32_t nc = (md.need[c / 2] >> (16 * (c % 2))) & 0xFFFFu;
+                        asm volatile("" : "+s"(nc));
+                        if (nc == 0xFFFFu) {  // uniform: a branch per chunk keeps one
+                            // chunk of rows live at a time (the reads of every chunk
+                            // ahead of the counts spill)
+#pragma unroll
+                            for (int w = 0; w < 4; w++) sel[4 * c + w] = 0;
+                            continue;
+                        }
+                    }
                     uint32_t m[NP][4];
 #pragma unroll
                     for (int p = 0; p < NP; p++) load_words<4>(row[p] + c * plane, m[p]);
@@ -115,10 +127,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? (NK <= 12 && N
 #pragma unroll
                         for (int w = 0; w < 4; w++) {
                             const uint32_t cw = 4 * c + w;
-                            // the branching comparison: its branches keep one chunk
-                            // of rows live at a time (without them the reads of every
-                            // chunk move ahead of the counts and spill)
-                            sel[cw] = at_least<NP, 4, false>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
+                            sel[cw] = at_least<NP, 4, true>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
                         }
                     } else {
                         uint32_t any[4], two[4];
