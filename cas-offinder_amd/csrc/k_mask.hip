@@ -127,7 +127,7 @@ __global__ __launch_bounds__(kThreads * G, NK >= kMaskWideWords ? (NK <= 12 && N
 #pragma unroll
                         for (int w = 0; w < 4; w++) {
                             const uint32_t cw = 4 * c + w;
-                            sel[cw] = at_least<NP, 4, true>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
+                            sel[cw] = at_least<NP, 4>(m, w, (md.need[cw / 8] >> (4 * (cw % 8))) & 15u);
                         }
                     } else {
                         uint32_t any[4], two[4];

@@ -742,7 +742,7 @@ constexpr uint32_t kXor3 = 0x96;  // a ^ b ^ c
 // Bits of word w set in at least `need` (uniform) of the NP rows: the
 // bit-sliced count b3 b2 b1 b0 of the set rows from full adders (v_bitop3
 // XOR3 and majority), then the comparison with need.
-template <int NP, int CH, bool BRANCHLESS = true>
+template <int NP, int CH>
 __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w, uint32_t need) {
     static_assert(NP == 4 || NP == 6 || NP == 8 || NP == 10 || NP == 12, "row counts 4, 6, 8, 10, 12");
     auto fa = [](uint32_t x, uint32_t y, uint32_t z, uint32_t &sum, uint32_t &carry) {
@@ -792,23 +792,6 @@ __device__ __forceinline__ uint32_t at_least(const uint32_t (&m)[NP][CH], int w,
             fa(c4, k1, k2, y2, d2);
             b1 = y1 ^ y2;
             fa(d1, d2, y1 & y2, b2, b3);  // weight 4 (at most 12 rows: b3 is weight 8)
-        }
-    }
-    if constexpr (!BRANCHLESS) {
-        switch (need) {  // uniform: a scalar branch per word
-            case 1: return b0 | __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
-            case 2: return __builtin_amdgcn_bitop3_b32(b1, b2, b3, kOr3);
-            case 3: return __builtin_amdgcn_bitop3_b32(b1, b0, b2 | b3, kAndOr);
-            case 4: return b2 | b3;
-            case 5: return __builtin_amdgcn_bitop3_b32(b2, b1 | b0, b3, kAndOr);
-            case 6: return __builtin_amdgcn_bitop3_b32(b2, b1, b3, kAndOr);
-            case 7: return __builtin_amdgcn_bitop3_b32(b2, b1 & b0, b3, kAndOr);
-            case 8: return b3;
-            case 9: return b3 & __builtin_amdgcn_bitop3_b32(b0, b1, b2, kOr3);
-            case 10: return b3 & (b1 | b2);
-            case 11: return b3 & __builtin_amdgcn_bitop3_b32(b1, b0, b2, kAndOr);
-            case 12: return b3 & b2;
-            default: return 0u;
         }
     }
     // count >= need, bit-sliced from the low bit up, without branches: with
