@@ -54,6 +54,7 @@ int search_plain(Lane &L, const DevGenome &dg, uint64_t slab_positions, const st
     // read the batched tables from it, so it must not be replayed without a
     // new upload
     L.graph_key.clear();
+    L.graph_plan = 0;
     L.launches.clear();
     HIP_TRY(L.d_keys.reserve(kw.size() + 16), "hipMalloc(keys)");
     if (!kw.empty())
@@ -523,8 +524,10 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     L.stats.t_plan_ms = 0;
     if (!(L.plan && L.plan->qs == qs && L.plan->gbase == gbase)) {
         const auto tp0 = std::chrono::steady_clock::now();
+        static std::atomic<uint64_t> next_serial{1};
         auto P = std::make_shared<BatchPlan>();
         if (const int rc = build_plan(qs, gbase, *P, err)) return rc;
+        P->serial = next_serial.fetch_add(1);
         P->qs = qs;
         P->gbase = gbase;
         L.plan = std::move(P);
@@ -605,10 +608,15 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             put(&bioff[g], sizeof bioff[g]);
             put(&bsoff[g], sizeof bsoff[g]);
         }
-        put(kw.data(), kw.size() * 4);
+        const size_t nkw = kw.size();
+        put(&nkw, sizeof nkw);
         put(ord.data(), ord.size() * sizeof ord[0]);
     }
-    const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key;
+    // the tables the graphs read: those of the same plan object (its tables
+    // never change), else compared word by word (new query objects with the
+    // same tables still replay)
+    const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key && (L.graph_plan == P.serial || L.graph_kw == kw);
+    if (replay) L.graph_plan = P.serial;
     if (!replay && !kw.empty())
         HIP_TRY(hipMemcpyAsync(d_keys, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
     // counters: [0] hits, candidate slots per launch (run_search sized them for one launch per query)
@@ -771,6 +779,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
         }
         if (e == hipSuccess) {
             L.graph_key = gkey;
+            L.graph_plan = P.serial;
+            L.graph_kw = kw;
             L.graph_variants = variant;
             L.graph_kernels = kernels;
         } else {
@@ -779,7 +789,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             L.graphs = false;
         }
     }
-    const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey;
+    const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey && L.graph_plan == P.serial;
     L.stats.graph = use_graph ? (replay ? 2 : 1) : 0;
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
     // launch i between its timing events (and, collecting per launch, the
@@ -895,6 +905,8 @@ void Lane::drop_graphs() {
     for (hipGraphExec_t x : graph) (void)hipGraphExecDestroy(x);
     graph.clear();
     graph_key.clear();
+    graph_plan = 0;
+    graph_kw.clear();
     graph_kernels.clear();
 }
 

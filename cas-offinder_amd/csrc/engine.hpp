@@ -190,6 +190,7 @@ struct DevBuf {
 // lane for the query objects it was built from (it holds them) and reused
 // when the next search names the same ones.
 struct BatchPlan {
+    uint64_t serial = 0;  // unique per plan built (the tables never change after)
     std::vector<std::shared_ptr<Query>> qs;
     std::vector<uint32_t> gbase;
     std::vector<KeyTable> ktab;
@@ -248,7 +249,9 @@ struct Lane {
     // (events between them time each launch), replayed when a search has the
     // same tables, kernel arguments and buffers
     std::vector<hipGraphExec_t> graph;
-    std::string graph_key;
+    std::string graph_key;           // kernel arguments and buffers of the captured sequence
+    uint64_t graph_plan = 0;         // serial of the plan the tables were last matched with
+    std::vector<uint32_t> graph_kw;  // the tables the graphs read (uploaded before the capture)
     std::vector<int> graph_variants;
     std::vector<std::string> graph_kernels;
     void drop_graphs();
