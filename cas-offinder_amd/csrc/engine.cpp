@@ -615,7 +615,8 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
     // the tables the graphs read: those of the same plan object (its tables
     // never change), else compared word by word (new query objects with the
     // same tables still replay)
-    const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key && (L.graph_plan == P.serial || L.graph_kw == kw);
+    const bool replay = graphs && !L.graph.empty() && gkey == L.graph_key &&
+                        ((P.serial && L.graph_plan == P.serial) || L.graph_kw == kw);
     if (replay) L.graph_plan = P.serial;
     if (!replay && !kw.empty())
         HIP_TRY(hipMemcpyAsync(d_keys, kw.data(), kw.size() * 4, hipMemcpyHostToDevice, L.stream), "hipMemcpy(keys)");
@@ -789,7 +790,7 @@ int search_batched(Lane &L, const DevGenome &dg, const std::vector<std::shared_p
             L.graphs = false;
         }
     }
-    const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey && L.graph_plan == P.serial;
+    const bool use_graph = graphs && !L.graph.empty() && L.graph_key == gkey && P.serial && L.graph_plan == P.serial;
     L.stats.graph = use_graph ? (replay ? 2 : 1) : 0;
     HIP_TRY(hipEventRecord(L.ev[0], L.stream), "hipEventRecord");
     // launch i between its timing events (and, collecting per launch, the

@@ -121,6 +121,52 @@ def test_plain_search_between_batched_searches(tmp_path):
         ctx.close()
 
 
+def _mutate_guides(text):
+    """The same input with the first A/C/G/T base of every guide changed:
+    tables of the same shapes (launch arguments alike) but other contents."""
+    lines = text.split("\n")
+    out = lines[:2]
+    for l in lines[2:]:
+        if not l.strip():
+            out.append(l)
+            continue
+        g, rest = l.split(" ", 1)
+        j = next(i for i, c in enumerate(g) if c in "ACGT")
+        out.append(g[:j] + "ACGT"["ACGT".index(g[j]) - 1] + g[j + 1:] + " " + rest)
+    return "\n".join(out)
+
+
+def test_replay_with_new_query_objects(tmp_path):
+    """Graphs replay for the same query objects (the plan's tables cannot
+    have changed) and for new objects whose tables equal the captured ones
+    word for word; new objects with other tables (guides of one changed
+    base: the same table shapes and launch arguments) recapture, and so does
+    the first set searched again after them.  Every search equals an eager
+    context's."""
+    gp = str(tmp_path / "g.fa")
+    casoff.synth_write(casoff.synth_spec(seed=25, n_contigs=4, total_len=1_500_000, n_frac=0.01), gp, 0)
+    texts_a = [t for _, t in workloads.group_inputs(workloads.mixed_guides(8, 48, 3, 6, 0.5, 1), gp)]
+    texts_b = [_mutate_guides(t) for t in texts_a]
+    eager, graph = _context(False), _context(True)
+    try:
+        for c in (eager, graph):
+            c.load(gp)
+        want_a = eager.search([eager.query(t) for t in texts_a], casoff.MODE_BATCHED).records()
+        want_b = eager.search([eager.query(t) for t in texts_b], casoff.MODE_BATCHED).records()
+        assert len(want_a) > 0 and len(want_b) > 0
+        qa = [graph.query(t) for t in texts_a]
+        steps = [(qa, want_a, 1), (qa, want_a, 2), ([graph.query(t) for t in texts_a], want_a, 2),
+                 ([graph.query(t) for t in texts_b], want_b, 1), ([graph.query(t) for t in texts_b], want_b, 2),
+                 (qa, want_a, 1), (qa, want_a, 2)]
+        for i, (qs, want, mode) in enumerate(steps):
+            got = graph.search(qs, casoff.MODE_BATCHED).records()
+            assert len(got) == len(want) and (got == want).all(), "step %d" % i
+            assert graph.stats()["graph"] == mode, ("step %d" % i, graph.stats()["graph"])
+    finally:
+        eager.close()
+        graph.close()
+
+
 def _ctx_with_env(env):
     """A context opened with the variables of `env` set (lanes read them at open)."""
     old = {k: os.environ.get(k) for k in env}
