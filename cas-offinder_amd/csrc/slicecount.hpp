@@ -88,12 +88,21 @@ COF_SLICE_FN void slice_count(uint32_t (&v)[N], uint32_t (&c)[6]) {
     for (int w = 0; w < 6; w++) c[w] = P.out[w] >= 0 ? v[P.out[w] >= 0 ? P.out[w] : 0] : 0u;
 }
 
+// Count planes that can be set for a count of at most n: the bit width of n.
+constexpr int slice_bits(int n) { return n <= 0 ? 0 : 1 + slice_bits(n >> 1); }
+
 // z = x + y + cin, bit-sliced (six planes, cin one plane; the sum must stay
-// below 64): a ripple of full adders.
+// below 64): a ripple of full adders over the B low planes, where both x
+// and y are zero from plane B on (the default: all six).  The adders stop
+// there: the carry is plane B and the planes above it are zero (the
+// compiler does not fold v_bitop3 with zero operands, so adders over known
+// zero planes would be issued).
+template <int B = 6>
 COF_SLICE_FN void slice_add(const uint32_t (&x)[6], const uint32_t (&y)[6], uint32_t (&z)[6], uint32_t cin = 0) {
+    static_assert(B >= 1 && B <= 6, "one to six planes");
     uint32_t carry = cin;
 #pragma unroll
-    for (int w = 0; w < 6; w++) {
+    for (int w = 0; w < B; w++) {
         const uint32_t a = x[w], b = y[w];
 #if defined(__HIP_DEVICE_COMPILE__)
         z[w] = __builtin_amdgcn_bitop3_b32(a, b, carry, 0x96);
@@ -103,6 +112,8 @@ COF_SLICE_FN void slice_add(const uint32_t (&x)[6], const uint32_t (&y)[6], uint
         carry = (a & b) | (carry & (a | b));
 #endif
     }
+#pragma unroll
+    for (int w = B; w < 6; w++) z[w] = w == B ? carry : 0u;
 }
 
 // Lanes whose count c is at most t (both six bit planes): c <= t exactly
@@ -192,12 +203,15 @@ COF_SLICE_FN void slice_halves(const uint32_t (&m0)[KN], Mu mu, uint32_t kn, con
         if constexpr (RV) slice_count<KN - S - 1>(py, y1);
     }
     slice_count<KN - S - 1>(pe, e1);
+    // every partial count is of at most S + 1 planes: its planes from
+    // slice_bits(S + 1) on are zero
+    constexpr int B = slice_bits(S + 1);
     if constexpr (RV) {
-        slice_add(a1, y1, h1);
-        slice_add(c0, e1, h2, bs);
+        slice_add<B>(a1, y1, h1);
+        slice_add<B>(c0, e1, h2, bs);
     } else {
-        slice_add(a1, c0, h1);
-        slice_add(x1, e1, h2, bs);
+        slice_add<B>(a1, c0, h1);
+        slice_add<B>(x1, e1, h2, bs);
     }
 }
 
