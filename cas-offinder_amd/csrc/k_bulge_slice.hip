@@ -142,13 +142,22 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                 WH[p] = (uint32_t)__builtin_amdgcn_sbfe((int32_t)fh[p / 32], p % 32, 1);
             }
             const uint32_t live = valid[0] ? ~0u : 0u;
+            // masks of the last three care positions (all ones inside the run),
+            // held in VGPRs: as uniform conditions they became spilled lane
+            // masks read back with v_readlane in every word
+            uint32_t keep[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) {
+                keep[j] = (uint32_t)(KN - 3 + j) < sd.kn ? ~0u : 0u;
+                if (CUT) asm volatile("" : "+v"(keep[j]));
+            }
             for (uint32_t w = 0; w < sd.nwords; w++) {
                 const kptr_t blk = blocks + w * sd.wstride;
                 uint32_t M0[KN];
 #pragma unroll
                 for (int k = 0; k < KN; k++) {
                     M0[k] = __builtin_amdgcn_bitop3_b32(blk[k], WL[k + OFF], blk[KN + k] ^ WH[k + OFF], kXorOr);
-                    if (CUT && k >= KN - 3 && (uint32_t)k >= sd.kn) M0[k] = 0;  // positions past the care run
+                    if (CUT && k >= KN - 3) M0[k] &= keep[k - (KN - 3)];  // positions past the care run
                 }
                 uint32_t pass = 0;
                 const bool fine = (blk[2 * KN + (kSliceTPlanes + 1) * NF] & 1u) != 0;  // uniform
@@ -180,8 +189,8 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                         // each half of the family's split range bounded on its own
                         // (slice_halves), M_0's count shared by the families in c0
                         uint32_t h1[6], h2[6];
-                        if (a.rev) slice_halves<KN, true, CUT>(M0, mu, sd.kn, c0, h1, h2);
-                        else slice_halves<KN, false, CUT>(M0, mu, sd.kn, c0, h1, h2);
+                        if (a.rev) slice_halves<KN, true, CUT>(M0, mu, keep, c0, h1, h2);
+                        else slice_halves<KN, false, CUT>(M0, mu, keep, c0, h1, h2);
                         le = slice_le(h1, t) | slice_le(h2, t);
                     }
                     pass |= le & tp[kSliceTPlanes];

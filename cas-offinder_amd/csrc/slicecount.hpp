@@ -150,7 +150,8 @@ COF_SLICE_FN uint32_t slice_le(const uint32_t (&c)[6], const uint32_t (&t)[6]) {
 // it once (forward: M_0 over (S, KN); reversed: M_0 over [0, S)).
 // Positions from kn on (KN - 4 < kn <= KN) lie past the guide's care run and
 // count nothing: m0 is zero there, and mu(k) is masked where it enters a
-// count alone (CUT: kn may be below KN; without it kn = KN, as for the
+// count alone (CUT: kn may be below KN, keep[j] is all ones where position
+// KN - 3 + j is inside the run, else zero; without CUT kn = KN, as for the
 // kernel's widths 21 and 23).
 template <int KN, bool RV>
 COF_SLICE_FN void slice_half_shared(const uint32_t (&m0)[KN], uint32_t (&c0)[6]) {
@@ -171,7 +172,7 @@ COF_SLICE_FN void slice_half_shared(const uint32_t (&m0)[KN], uint32_t (&c0)[6])
 // h1 / h2: count planes of the two half bounds of the family whose mismatch
 // plane at care position k is mu(k); c0 from slice_half_shared.
 template <int KN, bool RV, bool CUT, typename Mu>
-COF_SLICE_FN void slice_halves(const uint32_t (&m0)[KN], Mu mu, uint32_t kn, const uint32_t (&c0)[6],
+COF_SLICE_FN void slice_halves(const uint32_t (&m0)[KN], Mu mu, const uint32_t (&keep)[3], const uint32_t (&c0)[6],
                                uint32_t (&h1)[6], uint32_t (&h2)[6]) {
     constexpr int S = KN / 2;
     uint32_t pa[S + 1], a1[6];  // b at k <= S
@@ -197,7 +198,7 @@ COF_SLICE_FN void slice_halves(const uint32_t (&m0)[KN], Mu mu, uint32_t kn, con
         for (int k = S + 1; k < KN; k++) {
             uint32_t m = mu(k);
             pe[k - S - 1] = m0[k] & m;
-            if (CUT && k >= KN - 3 && (uint32_t)k >= kn) m = 0;  // past the care run (kn > KN - 4)
+            if (CUT && k >= KN - 3) m &= keep[k - (KN - 3)];  // past the care run (kn > KN - 4)
             py[k - S - 1] = m;
         }
         if constexpr (RV) slice_count<KN - S - 1>(py, y1);

@@ -97,7 +97,9 @@ static bool check_halves(std::mt19937_64 &rng) {
         }
         uint32_t c0[6], h1[6], h2[6];
         slice_half_shared<KN, RV>(m0, c0);
-        slice_halves<KN, RV, CUT>(m0, [&](int k) { return mu[k]; }, (uint32_t)kn, c0, h1, h2);
+        uint32_t keep[3];
+        for (int j = 0; j < 3; j++) keep[j] = KN - 3 + j < kn ? ~0u : 0u;
+        slice_halves<KN, RV, CUT>(m0, [&](int k) { return mu[k]; }, keep, c0, h1, h2);
         uint32_t mu_in[KN], b[KN];
         for (int k = 0; k < KN; k++) {
             mu_in[k] = k < kn ? mu[k] : 0u;
