@@ -77,14 +77,16 @@ constexpr uint32_t kBulgeIndexMaxBytes = 32 * 1024;  // static LDS + index stays
 // per filter family f (dn window shifts -1..-dn, then dp shifts +1..+dp) six
 // threshold planes T[0..5] and an active word: family f's count
 // popc(M0 & Mf) over the care run must be at most T(g) for slot g to reach
-// the exact counts; then a flags dword (bit 0: the word's bounds are large
-// enough that each family is bounded on two halves of its split range
-// instead of the whole range).  Slots are ordered by decreasing bound, so
+// the exact counts; then a flags dword (bit f: family f's bounds in the word
+// are large enough that it is bounded on two halves of its split range
+// instead of the whole range; both are lower bounds of every split, so the
+// choice only moves work between the filter and the exact counts).  Slots
+// are ordered by decreasing bound, so
 // that a word holds guides of similar thresholds.  After the blocks, per slot
 // kBulgeSlotWords words [glo, ghi, variant block offset, family filter bias
 // per alignment].
 constexpr int kSliceTPlanes = 6;
-constexpr int kSliceFineT = 6;  // smallest bound of a word bounded on halves
+constexpr int kSliceFineT = 6;  // smallest bound of a family bounded on halves
 constexpr uint32_t kSliceMaxGuides = 256;
 struct SliceDesc {
     uint32_t nslots, nwords;  // guides (slots) and 32-slot words

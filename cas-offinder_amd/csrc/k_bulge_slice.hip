@@ -20,8 +20,8 @@
 // guides has.  The
 // counts of 32 guides come from one carry-save adder network over the kn
 // planes (slicecount.hpp) and one bit-sliced comparison with the guides'
-// bounds.  For words of large bounds (slots are sorted by bound) each family
-// is bounded on the two halves of its split range instead, like
+// bounds.  A family whose bounds in the word are large (slots are sorted
+// by bound) is bounded on the two halves of its split range instead, like
 // bulge_family's segments: far fewer pairs then reach the exact counts.
 // A genome N reads as base T here (no non-ACGT plane): a count can
 // only drop, so the filter stays a lower bound.  The (candidate, guide)
@@ -160,9 +160,10 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                     if (CUT && k >= KN - 3) M0[k] &= keep[k - (KN - 3)];  // positions past the care run
                 }
                 uint32_t pass = 0;
-                const bool fine = (blk[2 * KN + (kSliceTPlanes + 1) * NF] & 1u) != 0;  // uniform
+                // bit f: family f bounded on the halves of its split range (uniform)
+                const uint32_t fines = blk[2 * KN + (kSliceTPlanes + 1) * NF];
                 uint32_t c0[6] = {};
-                if (fine) {
+                if (fines) {
                     if (a.rev) slice_half_shared<KN, true>(M0, c0);
                     else slice_half_shared<KN, false>(M0, c0);
                 }
@@ -178,7 +179,7 @@ __global__ __launch_bounds__(kThreads, (slice_min_blocks<DN, DP, KN>())) void bu
                                                            kXorOr);
                     };
                     uint32_t le;
-                    if (!fine) {
+                    if (!((fines >> f) & 1u)) {
                         // every split: mismatches at both alignments, at most bound + gap of them
                         uint32_t b[KN], c[6];
 #pragma unroll

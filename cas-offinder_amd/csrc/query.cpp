@@ -869,7 +869,8 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
                 if ((T >> p) & 1) fp[p] |= bit;
             fp[kSliceTPlanes] |= bit;
         }
-        if (top(r) >= kSliceFineT) blk[2 * d.KN + (kSliceTPlanes + 1) * nf] |= 1u;
+        for (int f = 0; f < nf; f++)
+            if (bound[r][f] >= kSliceFineT) blk[2 * d.KN + (kSliceTPlanes + 1) * nf] |= 1u << f;
         uint32_t *slot = &s.words[d.slot_off + (size_t)kBulgeSlotWords * si];
         slot[0] = rec[0];
         slot[1] = rec[1];
