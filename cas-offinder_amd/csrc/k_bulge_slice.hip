@@ -16,7 +16,8 @@
 // threshold + gap, and the no-bulge key (popc(M_0) <= its threshold) is
 // covered by the count of the guide's first family, which can only be
 // smaller.  Slots are grouped by their family sets (a table may merge
-// queries of different bulge sizes) and a word skips a family none of its
+// queries of different bulge sizes; a set may start a new word, leaving
+// empty slots that never pass) and a word skips a family none of its
 // guides has.  The
 // counts of 32 guides come from one carry-save adder network over the kn
 // planes (slicecount.hpp) and one bit-sliced comparison with the guides'

@@ -851,9 +851,53 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
     std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
         return fams(x) != fams(y) ? fams(x) > fams(y) : top(x) > top(y);
     });
-    s.words.assign(d.slot_off + (size_t)kBulgeSlotWords * n, 0);
-    for (uint32_t si = 0; si < n; si++) {  // slot si holds guide record order[si]
-        const uint32_t r = order[si];
+    // word layout: the guides in this order packed densely, or each family
+    // set starting a new word (empty slots have no family and never pass),
+    // so that no word evaluates a family for guides that lack it; the layout
+    // of the smaller estimated filter cost per candidate, in VALU per word
+    // from the kernel's code: ~93 for the alignment-0 mismatches and the
+    // queue, 13 more for the shared count of a word with a family bounded on
+    // halves, and per family it evaluates 106 (whole split range) or 140
+    // (two halves)
+    auto layout = [&](bool pure, std::vector<uint32_t> &slot) {
+        uint32_t si = 0;
+        for (uint32_t i = 0; i < n; i++) {
+            if (pure && i && fams(order[i]) != fams(order[i - 1])) si = (si + 31) & ~31u;
+            slot[i] = si++;
+        }
+        return si;  // slots
+    };
+    auto cost = [&](const std::vector<uint32_t> &slot, uint32_t nsl) {
+        const uint32_t nw = (nsl + 31) / 32;
+        std::vector<uint32_t> wf(nw, 0), wfine(nw, 0);
+        for (uint32_t i = 0; i < n; i++) {
+            const uint32_t w = slot[i] / 32, r = order[i];
+            wf[w] |= fams(r);
+            for (int f = 0; f < nf; f++)
+                if (bound[r][f] >= kSliceFineT) wfine[w] |= 1u << f;
+        }
+        uint64_t c = 0;
+        for (uint32_t w = 0; w < nw; w++) {
+            c += 93 + (wfine[w] ? 13 : 0);
+            for (int f = 0; f < nf; f++)
+                if ((wf[w] >> f) & 1) c += (wfine[w] >> f) & 1 ? 140 : 106;
+        }
+        return c;
+    };
+    std::vector<uint32_t> slot_of(n), pure_slot(n);
+    uint32_t nsl = layout(false, slot_of);
+    const uint32_t pure_nsl = layout(true, pure_slot);
+    if (pure_nsl <= kSliceMaxGuides && cost(pure_slot, pure_nsl) < cost(slot_of, nsl)) {
+        slot_of.swap(pure_slot);
+        nsl = pure_nsl;
+    }
+    d.nslots = nsl;
+    d.nwords = (nsl + 31) / 32;
+    d.slot_off = d.nwords * d.wstride;
+    if (!slice_desc_valid(d, ns)) return false;
+    s.words.assign(d.slot_off + (size_t)kBulgeSlotWords * nsl, 0);
+    for (uint32_t i = 0; i < n; i++) {  // slot slot_of[i] holds guide record order[i]
+        const uint32_t si = slot_of[i], r = order[i];
         const uint32_t *rec = &t.words[kClassHdr + (size_t)r * S];
         uint32_t *blk = &s.words[(size_t)(si / 32) * d.wstride];
         const uint32_t bit = 1u << (si % 32);
