@@ -846,32 +846,31 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
         for (int f = 0; f < nf; f++) m |= (bound[r][f] >= 0 ? 1u : 0u) << f;
         return m;
     };
-    std::vector<uint32_t> order(n);
-    for (uint32_t r = 0; r < n; r++) order[r] = r;
-    std::stable_sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-        return fams(x) != fams(y) ? fams(x) > fams(y) : top(x) > top(y);
-    });
-    // word layout: the guides in this order packed densely, or each family
-    // set starting a new word (empty slots have no family and never pass),
-    // so that no word evaluates a family for guides that lack it; the layout
-    // of the smaller estimated filter cost per candidate, in VALU per word
-    // from the kernel's code: ~93 for the alignment-0 mismatches and the
-    // queue, 13 more for the shared count of a word with a family bounded on
-    // halves, and per family it evaluates 106 (whole split range) or 140
-    // (two halves)
-    auto layout = [&](bool pure, std::vector<uint32_t> &slot) {
+    // word layout: guides grouped by family set, each set ordered by bound
+    // (either direction), the sets in some order, packed densely or each set
+    // starting a new word (empty slots have no family and never pass); the
+    // layout of the smallest estimated filter cost per candidate, in VALU
+    // per word from the kernel's code: ~93 for the alignment-0 mismatches
+    // and the queue, 13 more for the shared count of a word with a family
+    // bounded on halves, and per family it evaluates 106 (whole split range)
+    // or 140 (two halves).  Any layout gives the same records.
+    std::vector<uint32_t> sets;  // family sets present, largest first
+    for (uint32_t r = 0; r < n; r++)
+        if (std::find(sets.begin(), sets.end(), fams(r)) == sets.end()) sets.push_back(fams(r));
+    std::sort(sets.rbegin(), sets.rend());
+    auto layout = [&](const std::vector<uint32_t> &ord, bool pure, std::vector<uint32_t> &slot) {
         uint32_t si = 0;
         for (uint32_t i = 0; i < n; i++) {
-            if (pure && i && fams(order[i]) != fams(order[i - 1])) si = (si + 31) & ~31u;
+            if (pure && i && fams(ord[i]) != fams(ord[i - 1])) si = (si + 31) & ~31u;
             slot[i] = si++;
         }
         return si;  // slots
     };
-    auto cost = [&](const std::vector<uint32_t> &slot, uint32_t nsl) {
+    auto cost = [&](const std::vector<uint32_t> &ord, const std::vector<uint32_t> &slot, uint32_t nsl) {
         const uint32_t nw = (nsl + 31) / 32;
         std::vector<uint32_t> wf(nw, 0), wfine(nw, 0);
         for (uint32_t i = 0; i < n; i++) {
-            const uint32_t w = slot[i] / 32, r = order[i];
+            const uint32_t w = slot[i] / 32, r = ord[i];
             wf[w] |= fams(r);
             for (int f = 0; f < nf; f++)
                 if (bound[r][f] >= kSliceFineT) wfine[w] |= 1u << f;
@@ -884,13 +883,39 @@ bool build_bulge_slices(const BulgeTable &t, const PatternDesc &pd, SliceIndex &
         }
         return c;
     };
-    std::vector<uint32_t> slot_of(n), pure_slot(n);
-    uint32_t nsl = layout(false, slot_of);
-    const uint32_t pure_nsl = layout(true, pure_slot);
-    if (pure_nsl <= kSliceMaxGuides && cost(pure_slot, pure_nsl) < cost(slot_of, nsl)) {
-        slot_of.swap(pure_slot);
-        nsl = pure_nsl;
-    }
+    std::vector<uint32_t> order, slot_of;
+    uint32_t nsl = 0;
+    uint64_t best = UINT64_MAX;
+    std::vector<uint32_t> perm(sets.size());
+    for (size_t i = 0; i < perm.size(); i++) perm[i] = (uint32_t)i;
+    do {  // set orders (from largest-first) x bound directions x dense / padded
+        for (uint32_t dirs = 0; dirs < (1u << sets.size()); dirs++) {
+            auto rank = [&](uint32_t r) {
+                for (size_t i = 0; i < perm.size(); i++)
+                    if (sets[perm[i]] == fams(r)) return (uint32_t)i;
+                return 0u;
+            };
+            std::vector<uint32_t> ord(n);
+            for (uint32_t r = 0; r < n; r++) ord[r] = r;
+            std::stable_sort(ord.begin(), ord.end(), [&](uint32_t x, uint32_t y) {
+                const uint32_t rx = rank(x), ry = rank(y);
+                if (rx != ry) return rx < ry;
+                return (dirs >> rx) & 1 ? top(x) < top(y) : top(x) > top(y);
+            });
+            for (const bool pure : {false, true}) {
+                std::vector<uint32_t> slot(n);
+                const uint32_t ns_ = layout(ord, pure, slot);
+                if (ns_ > kSliceMaxGuides) continue;
+                const uint64_t c = cost(ord, slot, ns_);
+                if (c < best) {
+                    best = c;
+                    order = ord;
+                    slot_of = slot;
+                    nsl = ns_;
+                }
+            }
+        }
+    } while (std::next_permutation(perm.begin(), perm.end()));
     d.nslots = nsl;
     d.nwords = (nsl + 31) / 32;
     d.slot_off = d.nwords * d.wstride;
