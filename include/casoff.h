@@ -226,6 +226,8 @@ int cof_result_format(cof_ctx *ctx, const cof_result *r, int qi, const char *pat
  * context and genome it was searched on (else COF_E_INVALID). */
 /* Same records into a malloc-ed buffer (caller frees with cof_free). */
 int cof_result_format_mem(cof_ctx *ctx, const cof_result *r, int qi, char **out, size_t *len);
+/* Releases a result; one whose hit array holds 64 MiB or more is released
+ * on a short-lived thread of the library (the call returns at once). */
 int cof_result_free(cof_result *r);
 /* Formats caller-supplied hits of one query (hit.query ignored) against the
  * context's genome, exactly as cof_result_format_mem; no header lines. */
